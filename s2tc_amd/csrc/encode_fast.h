@@ -1,0 +1,304 @@
+// encode_fast.h - quick-mode ("MODE_FAST") block encoder, one lane per block.
+//
+// Follows s2tc_encode_block<dxt, ColorDist, MODE_FAST, refine>
+// (s2tc_algorithm.cpp:872-935 and :1009-1107): darkest/brightest colour by
+// distance to black, DXT5 alpha min/max over alpha != 255 texels seeded from
+// texel (0,0), equal-endpoint fix, NEVER/ALWAYS/LOOP refinement, packing.
+// All sixteen texels of a block live in the lane's registers; texel k is at
+// x = k >> 2, y = k & 3, so unrolled loops over k visit texels x-major like
+// the reference (:900-901), which fixes the first-occurrence tie-breaks.
+#pragma once
+
+#include "s2tc_device.h"
+
+namespace s2tc {
+
+// An upper bound on REFINE_LOOP iterations that is never reached: the loop
+// only continues while the score strictly decreases, and each iteration's
+// endpoints are a function of the previous iteration's 2-way partition of at
+// most 16 texels, so at most 2^16 + 1 iterations can occur.
+#define S2TC_LOOP_CAP 65538
+
+struct BlockOut { uint32_t w[4]; };
+
+template <int CD, bool TRANS>
+struct ColorRefineLane {
+	typedef Dist<CD> D;
+	typedef typename D::F F;
+	// one s2tc_try_encode_block pass (:582-643) over the lane's texels
+	static S2TC_D uint32_t try_pass(const uint32_t (&tx)[16], const F (&f)[16], int w, int h, uint32_t c0,
+	                                uint32_t c1, uint32_t &score, int &n1, uint32_t &s1)
+	{
+		F e0 = D::feat(c0), e1 = D::feat(c1);
+		uint32_t bits = 0;
+		score = 0;
+		n1 = 0;
+		s1 = 0;
+#pragma unroll
+		for (int k = 0; k < 16; ++k) {
+			const int x = k >> 2, y = k & 3, slot = y * 4 + x;
+			const bool valid = x < w && y < h;
+			if (!valid) continue;
+			if (TRANS && texel_a(tx[k]) == 0) {
+				bits |= 3u << (2 * slot);
+				continue;
+			}
+			int d0 = D::d(f[k], e0), d1 = D::d(f[k], e1);
+			bool b = d1 < d0;
+			score += (uint32_t)(b ? d1 : d0);
+			uint32_t v = texel565(tx[k]);
+			uint32_t pk = (v >> 11) | (((v >> 5) & 63u) << 10) | ((v & 31u) << 21);
+			n1 += b ? 1 : 0;
+			s1 += b ? pk : 0u;
+			bits |= (b ? 1u : 0u) << (2 * slot);
+		}
+		return bits;
+	}
+};
+
+// s2tc_evaluate_colors_result_t<color_t, bigcolor_t, 1>::evaluate (:542-551)
+// from packed sums (r | g << 10 | b << 21).
+S2TC_D bool eval_color(int n0, uint32_t s0, int n1, uint32_t s1, uint32_t &c0, uint32_t &c1)
+{
+	if (!n0 && !n1) return false;
+	if (n0) {
+		int r = rmean((int)(s0 & 1023u), n0) & 31, g = rmean((int)((s0 >> 10) & 2047u), n0) & 63,
+		    b = rmean((int)(s0 >> 21), n0) & 31;
+		c0 = (uint32_t)((r << 11) | (g << 5) | b);
+	}
+	if (n1) {
+		int r = rmean((int)(s1 & 1023u), n1) & 31, g = rmean((int)((s1 >> 10) & 2047u), n1) & 63,
+		    b = rmean((int)(s1 >> 21), n1) & 31;
+		c1 = (uint32_t)((r << 11) | (g << 5) | b);
+	}
+	return true;
+}
+
+// equal fix and endpoint order of the colour refiners (:796-813)
+template <bool TRANS>
+S2TC_D void color_fix_order(uint32_t &bits, uint32_t &c0, uint32_t &c1)
+{
+	if (c0 == c1) {
+		c1 = c0 == 0xffffu ? c1 - 1u : c1 + 1u;
+		bits = keep_ones(bits);
+	}
+	if (TRANS ? c1 < c0 : c0 < c1) {
+		uint32_t t = c0;
+		c0 = c1;
+		c1 = t;
+		bits = swap_remap2(bits);
+	}
+}
+
+// alpha pass of s2tc_try_encode_block with have_0_255 (:582-643)
+S2TC_D uint64_t alpha_try_lane(const uint32_t (&tx)[16], int w, int h, int a0, int a1, uint32_t &score, int &n0,
+                               int &s0, int &n1, int &s1)
+{
+	uint64_t bits = 0;
+	score = 0;
+	n0 = s0 = n1 = s1 = 0;
+#pragma unroll
+	for (int k = 0; k < 16; ++k) {
+		const int x = k >> 2, y = k & 3, slot = y * 4 + x;
+		if (!(x < w && y < h)) continue;
+		int a = texel_a(tx[k]);
+		int d0 = adist(a, a0), d1 = adist(a, a1);
+		bool b = d1 < d0;
+		int bd = b ? d1 : d0;
+		int dz = a * a, df = (255 - a) * (255 - a);
+		uint32_t idx;
+		if (dz <= bd) {
+			idx = 6;
+			score += (uint32_t)dz;
+		} else if (df <= bd) {
+			idx = 7;
+			score += (uint32_t)df;
+		} else {
+			idx = b ? 1u : 0u;
+			score += (uint32_t)bd;
+			n1 += b ? 1 : 0;
+			s1 += b ? a : 0;
+			n0 += b ? 0 : 1;
+			s0 += b ? 0 : a;
+		}
+		bits |= (uint64_t)idx << (3 * slot);
+	}
+	return bits;
+}
+
+S2TC_D bool eval_alpha(int n0, int s0, int n1, int s1, int &a0, int &a1)
+{
+	if (!n0 && !n1) return false;
+	if (n0) a0 = rmean(s0, n0) & 255;
+	if (n1) a1 = rmean(s1, n1) & 255;
+	return true;
+}
+
+// alpha equal fix and order (:673-705)
+S2TC_D void alpha_fix_order(uint64_t &bits, int &a0, int &a1)
+{
+	if (a1 == a0) {
+		a1 = a0 == 255 ? a1 - 1 : a1 + 1;
+		bits = keep_not_one3(bits);
+	}
+	if (a1 < a0) {
+		int t = a0;
+		a0 = a1;
+		a1 = t;
+		bits = swap_remap3(bits);
+	}
+}
+
+// DXT5 alpha refinement NEVER/ALWAYS/LOOP (:645-765) on the lane's texels
+S2TC_D uint64_t alpha_refine_lane(const uint32_t (&tx)[16], int w, int h, int refine, int &a0, int &a1)
+{
+	uint32_t score;
+	int n0, s0, n1, s1;
+	if (refine == REFINE_NEVER) {
+		if (a1 < a0) {
+			int t = a0;
+			a0 = a1;
+			a1 = t;
+		}
+		return alpha_try_lane(tx, w, h, a0, a1, score, n0, s0, n1, s1);
+	}
+	uint64_t bits;
+	if (refine == REFINE_LOOP) {
+		bits = 0;
+		uint32_t s = 0x7fffffffu;
+		int na0 = a0, na1 = a1;
+		for (int it = 0; it < S2TC_LOOP_CAP; ++it) {
+			uint64_t b2 = alpha_try_lane(tx, w, h, na0, na1, score, n0, s0, n1, s1);
+			if (!(score < s)) break;
+			bits = b2;
+			s = score;
+			a0 = na0;
+			a1 = na1;
+			if (!eval_alpha(n0, s0, n1, s1, na0, na1)) break;
+		}
+	} else {
+		bits = alpha_try_lane(tx, w, h, a0, a1, score, n0, s0, n1, s1);
+		eval_alpha(n0, s0, n1, s1, a0, a1);
+	}
+	alpha_fix_order(bits, a0, a1);
+	return bits;
+}
+
+// colour refinement NEVER/ALWAYS/LOOP (:767-860) on the lane's texels
+template <int CD, bool TRANS>
+S2TC_D uint32_t color_refine_lane(const uint32_t (&tx)[16], const typename Dist<CD>::F (&f)[16], int w, int h,
+                                  int refine, uint32_t &c0, uint32_t &c1)
+{
+	typedef ColorRefineLane<CD, TRANS> R;
+	uint32_t score, s1;
+	int n1;
+	if (refine == REFINE_NEVER) {
+		if (TRANS ? c1 < c0 : c0 < c1) {
+			uint32_t t = c0;
+			c0 = c1;
+			c1 = t;
+		}
+		return R::try_pass(tx, f, w, h, c0, c1, score, n1, s1);
+	}
+	// totals over the texels that enter the accumulator, fixed per block
+	int ntot = 0;
+	uint32_t stot = 0;
+#pragma unroll
+	for (int k = 0; k < 16; ++k) {
+		const int x = k >> 2, y = k & 3;
+		bool use = x < w && y < h && !(TRANS && texel_a(tx[k]) == 0);
+		uint32_t v = texel565(tx[k]);
+		uint32_t pk = (v >> 11) | (((v >> 5) & 63u) << 10) | ((v & 31u) << 21);
+		ntot += use ? 1 : 0;
+		stot += use ? pk : 0u;
+	}
+	uint32_t bits;
+	if (refine == REFINE_LOOP) {
+		bits = 0;
+		uint32_t s = 0x7fffffffu, n0v = c0, n1v = c1;
+		for (int it = 0; it < S2TC_LOOP_CAP; ++it) {
+			uint32_t b2 = R::try_pass(tx, f, w, h, n0v, n1v, score, n1, s1);
+			if (!(score < s)) break;
+			bits = b2;
+			s = score;
+			c0 = n0v;
+			c1 = n1v;
+			if (!eval_color(ntot - n1, stot - s1, n1, s1, n0v, n1v)) break;
+		}
+	} else {
+		bits = R::try_pass(tx, f, w, h, c0, c1, score, n1, s1);
+		eval_color(ntot - n1, stot - s1, n1, s1, c0, c1);
+	}
+	color_fix_order<TRANS>(bits, c0, c1);
+	return bits;
+}
+
+// The whole quick-mode block.  tx[k] holds the quantized texel at x = k >> 2,
+// y = k & 3 (only x < w, y < h are read).
+template <int CD, int DXT>
+S2TC_D BlockOut encode_block_fast(const uint32_t (&tx)[16], int w, int h, int refine)
+{
+	typedef Dist<CD> D;
+	typedef typename D::F F;
+	const F black = D::feat(0u);
+	F f[16];
+	uint32_t c0 = 0xffffu, c1 = 0u;  // dummy (31,63,31) and (0,0,0) (:886-891)
+	int dmin = 0x7fffffff, dmax = 0;
+	int a0 = texel_a(tx[0]), a1 = a0;  // (:894-898)
+#pragma unroll
+	for (int k = 0; k < 16; ++k) {
+		const int x = k >> 2, y = k & 3;
+		const bool valid = x < w && y < h;
+		const uint32_t v = texel565(tx[k]);
+		f[k] = D::feat(v);
+		const bool use = valid && !(DXT == DXT1 && texel_a(tx[k]) == 0);
+		int d = D::d(f[k], black);
+		if (use && d > dmax) {
+			dmax = d;
+			c1 = v;
+		}
+		if (use && d < dmin) {
+			dmin = d;
+			c0 = v;
+		}
+		if (DXT == DXT5) {
+			int a = texel_a(tx[k]);
+			if (valid && a != 255) {
+				a1 = a > a1 ? a : a1;
+				a0 = a < a0 ? a : a0;
+			}
+		}
+	}
+	c1 = fix_equal565(c0, c1);
+	BlockOut o;
+	if (DXT == DXT1) {
+		uint32_t bits = color_refine_lane<CD, true>(tx, f, w, h, refine, c0, c1);
+		o.w[0] = c0 | (c1 << 16);
+		o.w[1] = bits;
+		o.w[2] = o.w[3] = 0;
+	} else if (DXT == DXT3) {
+		uint32_t bits = color_refine_lane<CD, false>(tx, f, w, h, refine, c0, c1);
+		uint64_t ab = 0;
+#pragma unroll
+		for (int k = 0; k < 16; ++k) {
+			const int x = k >> 2, y = k & 3;
+			if (x < w && y < h) ab |= (uint64_t)texel_a(tx[k]) << (4 * (y * 4 + x));
+		}
+		o.w[0] = (uint32_t)ab;
+		o.w[1] = (uint32_t)(ab >> 32);
+		o.w[2] = c0 | (c1 << 16);
+		o.w[3] = bits;
+	} else {
+		a1 = fix_equal_a(a0, a1);
+		uint32_t bits = color_refine_lane<CD, false>(tx, f, w, h, refine, c0, c1);
+		uint64_t ab = alpha_refine_lane(tx, w, h, refine, a0, a1);
+		uint64_t lo = (uint64_t)(a0 & 255) | ((uint64_t)(a1 & 255) << 8) | (ab << 16);
+		o.w[0] = (uint32_t)lo;
+		o.w[1] = (uint32_t)(lo >> 32);
+		o.w[2] = c0 | (c1 << 16);
+		o.w[3] = bits;
+	}
+	return o;
+}
+
+}  // namespace s2tc

@@ -1,0 +1,476 @@
+// encode_normal.h - MODE_NORMAL block encoder: all-16 and random candidates.
+//
+// Follows s2tc_encode_block<dxt, ColorDist, MODE_NORMAL, refine>
+// (s2tc_algorithm.cpp:936-1107).  One block is handled by one DPP row of 16
+// lanes; lane t owns texel t (x = t >> 2, y = t & 3, i.e. the reference's
+// x-major visiting order).  A 64-lane workgroup holds four blocks.
+//
+// Work split:
+//  * candidate gather: ballot + prefix count gives each opaque texel its
+//    position in the reference's candidate list (:940-951);
+//  * random candidates: lane t draws candidates t, t+16, ... with the
+//    counter-based generator (s2tc_prng.h) in the reference's draw order;
+//  * distance matrix: lane t computes rows i = 16p + t of dists[i][k]
+//    (:373-392) into LDS;
+//  * pair scan (reduce_colors_inplace, :393-410): for every row-block pair
+//    (p, q) lane t keeps its row of block p and its row of block q in
+//    registers and meets the 16 rows of block q by rotating them through the
+//    DPP row with row_ror; each pair's sum is formed once, by one lane;
+//  * argmin: the reference's fold "if (bestsum < 0 || sum < bestsum)" equals
+//    the first strict minimum over the pairs that come after the last pair
+//    with a negative (int32-wrapped, SRGB only) sum, or that last pair if no
+//    pair follows it.  Lanes reduce (sum, pair order) keys with DPP; only a
+//    block that has a negative sum runs the second pass.
+//  * refinement: lane t scores its own texel, the cluster sums are row
+//    reductions (4 DPP steps each).
+#pragma once
+
+#include "encode_fast.h"
+
+namespace s2tc {
+
+#define S2TC_RS 20  // LDS row stride in ints (16 values + 4 pad: 16-byte aligned, fewer bank conflicts)
+
+template <int CTRL> S2TC_D int dpp_i(int v) { return __builtin_amdgcn_mov_dpp(v, CTRL, 0xf, 0xf, false); }
+template <int CTRL> S2TC_D uint32_t dpp_u(uint32_t v) { return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, CTRL, 0xf, 0xf, false); }
+
+// all-reduce within a DPP row of 16 lanes: quad_perm [1,0,3,2], quad_perm
+// [2,3,0,1], row_half_mirror, row_mirror
+#define S2TC_ROW_STEPS(OP) OP(0xB1) OP(0x4E) OP(0x141) OP(0x140)
+S2TC_D uint32_t row_sum(uint32_t v)
+{
+#define S2TC_STEP(C) v += dpp_u<C>(v);
+	S2TC_ROW_STEPS(S2TC_STEP)
+#undef S2TC_STEP
+	return v;
+}
+S2TC_D uint32_t row_or(uint32_t v)
+{
+#define S2TC_STEP(C) v |= dpp_u<C>(v);
+	S2TC_ROW_STEPS(S2TC_STEP)
+#undef S2TC_STEP
+	return v;
+}
+S2TC_D int row_min(int v)
+{
+#define S2TC_STEP(C) { int o = dpp_i<C>(v); v = o < v ? o : v; }
+	S2TC_ROW_STEPS(S2TC_STEP)
+#undef S2TC_STEP
+	return v;
+}
+S2TC_D int row_max(int v)
+{
+#define S2TC_STEP(C) { int o = dpp_i<C>(v); v = o > v ? o : v; }
+	S2TC_ROW_STEPS(S2TC_STEP)
+#undef S2TC_STEP
+	return v;
+}
+// lexicographic (hi, lo) unsigned minimum
+S2TC_D void row_min_key(uint32_t &hi, uint32_t &lo)
+{
+#define S2TC_STEP(C) { uint32_t oh = dpp_u<C>(hi), ol = dpp_u<C>(lo); \
+	bool lt = oh < hi || (oh == hi && ol < lo); hi = lt ? oh : hi; lo = lt ? ol : lo; }
+	S2TC_ROW_STEPS(S2TC_STEP)
+#undef S2TC_STEP
+}
+
+struct ScanKey {
+	uint32_t hi, lo;  // (sum, pair order i*1024+j); all-ones = none
+	int qneg;         // largest pair order with a negative sum, -1 if none
+};
+
+// One rotation step of the pair scan: pairs (16p + t, 16q + partner) where
+// partner is the lane whose row of block q arrives via row_ror:R.
+template <int R, bool DIAG, int PASS, bool NEG>
+S2TC_D void scan_step(const int (&X)[16], const int (&Y)[16], int t, int p, int q, int m, int qthr, ScanKey &st)
+{
+	int s = 0;
+#pragma unroll
+	for (int k = 0; k < 16; ++k) {
+		int y = R ? dpp_i<0x120 + R>(Y[k]) : Y[k];
+		s = (int)((uint32_t)s + (uint32_t)(X[k] < y ? X[k] : y));  // int32 wrap as in the reference
+	}
+	const int b = R ? dpp_i<0x120 + R>(t) : t;
+	const int i = 16 * p + t, j = 16 * q + b;
+	bool valid = i < m && j < m;
+	if (DIAG && R == 8) valid = valid && t < 8;
+	const int lo = i < j ? i : j, hi = i < j ? j : i;
+	const uint32_t order = (uint32_t)(lo * 1024 + hi);
+	if (PASS == 2) valid = valid && (int)order > qthr;
+	const bool better = valid && ((uint32_t)s < st.hi || ((uint32_t)s == st.hi && order < st.lo));
+	st.hi = better ? (uint32_t)s : st.hi;
+	st.lo = better ? order : st.lo;
+	if (NEG && PASS == 1) {
+		const bool neg = valid && s < 0;
+		st.qneg = (neg && (int)order > st.qneg) ? (int)order : st.qneg;
+	}
+}
+
+template <int R, int END, bool DIAG, int PASS, bool NEG> struct ScanSteps {
+	static S2TC_D void run(const int (&X)[16], const int (&Y)[16], int t, int p, int q, int m, int qthr, ScanKey &st)
+	{
+		scan_step<R, DIAG, PASS, NEG>(X, Y, t, p, q, m, qthr, st);
+		ScanSteps<R + 1, END, DIAG, PASS, NEG>::run(X, Y, t, p, q, m, qthr, st);
+	}
+};
+template <int END, bool DIAG, int PASS, bool NEG> struct ScanSteps<END, END, DIAG, PASS, NEG> {
+	static S2TC_D void run(const int (&)[16], const int (&)[16], int, int, int, int, int, ScanKey &) {}
+};
+
+S2TC_D void load_row(const int *rows, int r, int (&X)[16])
+{
+	const int4 *p = (const int4 *)(rows + r * S2TC_RS);
+#pragma unroll
+	for (int k = 0; k < 4; ++k) {
+		int4 v = p[k];
+		X[4 * k] = v.x;
+		X[4 * k + 1] = v.y;
+		X[4 * k + 2] = v.z;
+		X[4 * k + 3] = v.w;
+	}
+}
+
+// Scan over all pairs i < j < m of the rows in LDS; pbw row blocks (uniform).
+template <int PASS, bool NEG>
+S2TC_D void pair_scan(const int *rows, int t, int m, int pbw, int qthr, ScanKey &st)
+{
+	for (int p = 0; p < pbw; ++p) {
+		int X[16];
+		load_row(rows, 16 * p + t, X);
+		for (int q = p; q < pbw; ++q) {
+			int Y[16];
+			load_row(rows, 16 * q + t, Y);
+			if (q == p)
+				ScanSteps<1, 9, true, PASS, NEG>::run(X, Y, t, p, q, m, qthr, st);
+			else
+				ScanSteps<0, 16, false, PASS, NEG>::run(X, Y, t, p, q, m, qthr, st);
+		}
+	}
+}
+
+// reduce_colors_inplace / _2fixpoints argmin over the rows: returns the
+// winning pair order i*1024+j (group-uniform).
+template <bool NEG>
+S2TC_D uint32_t reduce_rows(const int *rows, int t, int m, int pbw)
+{
+	ScanKey st{0xffffffffu, 0xffffffffu, -1};
+	pair_scan<1, NEG>(rows, t, m, pbw, -1, st);
+	if (NEG) {
+		const int q = row_max(st.qneg);
+		if (q >= 0) {
+			st.hi = st.lo = 0xffffffffu;
+			pair_scan<2, NEG>(rows, t, m, pbw, q, st);
+			row_min_key(st.hi, st.lo);
+			return st.lo == 0xffffffffu ? (uint32_t)q : st.lo;
+		}
+	}
+	row_min_key(st.hi, st.lo);
+	return st.lo;
+}
+
+// Per-group LDS: candidates (565 | alpha << 16), their features, the rows.
+template <int CD> struct GroupSmem {
+	uint32_t *cand;
+	typename Dist<CD>::F *feat;
+	int *rows;
+};
+
+template <int CD> S2TC_D size_t group_smem_bytes(int mc)
+{
+	return (size_t)mc * (sizeof(uint32_t) + sizeof(typename Dist<CD>::F) + S2TC_RS * sizeof(int));
+}
+
+template <int CD> S2TC_D GroupSmem<CD> group_smem(uint8_t *base, int g, int mc)
+{
+	uint8_t *p = base + (size_t)g * group_smem_bytes<CD>(mc);
+	GroupSmem<CD> s;
+	s.rows = (int *)p;
+	s.feat = (typename Dist<CD>::F *)(p + (size_t)mc * S2TC_RS * sizeof(int));
+	s.cand = (uint32_t *)(p + (size_t)mc * (S2TC_RS * sizeof(int) + sizeof(typename Dist<CD>::F)));
+	return s;
+}
+
+// Colour refinement on 16 lanes (:767-860).  Returns this lane's 2-bit index
+// (slot y*4+x), already remapped by the equal fix and the endpoint swap.
+template <int CD, bool TRANS>
+S2TC_D uint32_t color_refine_group(uint32_t tex, bool valid, const typename Dist<CD>::F &f, int refine,
+                                   uint32_t &c0, uint32_t &c1)
+{
+	typedef Dist<CD> D;
+	typedef typename D::F F;
+	const bool transp = TRANS && texel_a(tex) == 0;
+	const bool use = valid && !transp;
+	if (refine == REFINE_NEVER) {
+		if (TRANS ? c1 < c0 : c0 < c1) {
+			uint32_t tt = c0;
+			c0 = c1;
+			c1 = tt;
+		}
+		F e0 = D::feat(c0), e1 = D::feat(c1);
+		int d0 = D::d(f, e0), d1 = D::d(f, e1);
+		return !valid ? 0u : transp ? 3u : (d1 < d0 ? 1u : 0u);
+	}
+	const uint32_t v = texel565(tex);
+	const uint32_t pa = (v >> 11) | ((v & 31u) << 10) | (1u << 20);  // r | b << 10 | count << 20
+	const uint32_t pg = (v >> 5) & 63u;
+	const uint32_t atot = row_sum(use ? pa : 0u), gtot = row_sum(use ? pg : 0u);
+	uint32_t idx = 0;
+	uint32_t n0v = c0, n1v = c1, s = 0x7fffffffu;
+	for (int it = 0; it < S2TC_LOOP_CAP; ++it) {
+		F e0 = D::feat(n0v), e1 = D::feat(n1v);
+		int d0 = D::d(f, e0), d1 = D::d(f, e1);
+		const bool b = d1 < d0;
+		const uint32_t score = row_sum(use ? (uint32_t)(b ? d1 : d0) : 0u);
+		const uint32_t a1 = row_sum(use && b ? pa : 0u), g1 = row_sum(use && b ? pg : 0u);
+		const uint32_t idx2 = !valid ? 0u : transp ? 3u : (b ? 1u : 0u);
+		if (refine == REFINE_LOOP && !(score < s)) break;
+		idx = idx2;
+		s = score;
+		c0 = n0v;
+		c1 = n1v;
+		const int n1 = (int)(a1 >> 20), n0 = (int)(atot >> 20) - n1;
+		const uint32_t a0s = atot - a1, g0s = gtot - g1;
+		bool ok = (n0 | n1) != 0;
+		if (n0) {
+			int r = rmean((int)(a0s & 1023u), n0) & 31, g = rmean((int)g0s, n0) & 63,
+			    bb = rmean((int)((a0s >> 10) & 1023u), n0) & 31;
+			n0v = (uint32_t)((r << 11) | (g << 5) | bb);
+		}
+		if (n1) {
+			int r = rmean((int)(a1 & 1023u), n1) & 31, g = rmean((int)g1, n1) & 63,
+			    bb = rmean((int)((a1 >> 10) & 1023u), n1) & 31;
+			n1v = (uint32_t)((r << 11) | (g << 5) | bb);
+		}
+		if (refine != REFINE_LOOP) {  // ALWAYS: take the evaluated endpoints unchecked
+			c0 = n0v;
+			c1 = n1v;
+			break;
+		}
+		if (!ok) break;
+	}
+	if (c0 == c1) {
+		c1 = c0 == 0xffffu ? c1 - 1u : c1 + 1u;
+		idx = idx == 1u ? 1u : 0u;
+	}
+	if (TRANS ? c1 < c0 : c0 < c1) {
+		uint32_t tt = c0;
+		c0 = c1;
+		c1 = tt;
+		idx ^= (idx & 2u) ? 0u : 1u;
+	}
+	return idx;
+}
+
+// DXT5 alpha refinement on 16 lanes (:645-765); returns this lane's 3-bit index.
+S2TC_D uint32_t alpha_refine_group(int a, bool valid, int refine, int &a0, int &a1)
+{
+	if (refine == REFINE_NEVER) {
+		if (a1 < a0) {
+			int tt = a0;
+			a0 = a1;
+			a1 = tt;
+		}
+	}
+	uint32_t idx = 0;
+	int na0 = a0, na1 = a1;
+	uint32_t s = 0x7fffffffu;
+	for (int it = 0; it < S2TC_LOOP_CAP; ++it) {
+		int d0 = adist(a, na0), d1 = adist(a, na1);
+		bool b = d1 < d0;
+		int bd = b ? d1 : d0;
+		int dz = a * a, df = (255 - a) * (255 - a);
+		uint32_t i2;
+		int sc;
+		bool acc = false;
+		if (dz <= bd) {
+			i2 = 6;
+			sc = dz;
+		} else if (df <= bd) {
+			i2 = 7;
+			sc = df;
+		} else {
+			i2 = b ? 1u : 0u;
+			sc = bd;
+			acc = true;
+		}
+		if (!valid) {
+			i2 = 0;
+			sc = 0;
+			acc = false;
+		}
+		if (refine == REFINE_NEVER) return i2;
+		const uint32_t score = row_sum((uint32_t)sc);
+		// packed: sum a (<= 4080, 12 bits) | count << 13, per cluster
+		const uint32_t c1p = row_sum(acc && b ? ((uint32_t)a | (1u << 13)) : 0u);
+		const uint32_t c0p = row_sum(acc && !b ? ((uint32_t)a | (1u << 13)) : 0u);
+		if (refine == REFINE_LOOP && !(score < s)) break;
+		idx = i2;
+		s = score;
+		a0 = na0;
+		a1 = na1;
+		const int n0 = (int)(c0p >> 13), n1 = (int)(c1p >> 13);
+		if (n0) na0 = rmean((int)(c0p & 8191u), n0) & 255;
+		if (n1) na1 = rmean((int)(c1p & 8191u), n1) & 255;
+		if (refine != REFINE_LOOP) {
+			a0 = na0;
+			a1 = na1;
+			break;
+		}
+		if (!(n0 | n1)) break;
+	}
+	if (a1 == a0) {
+		a1 = a0 == 255 ? a1 - 1 : a1 + 1;
+		idx = idx == 1u ? 0u : idx;
+	}
+	if (a1 < a0) {
+		int tt = a0;
+		a0 = a1;
+		a1 = tt;
+		idx = idx == 0u ? 1u : idx == 1u ? 0u : (idx >= 6u ? idx : 7u - idx);
+	}
+	return idx;
+}
+
+// The whole MODE_NORMAL block on one DPP row.  tex: this lane's quantized
+// texel (0 when outside the block), t: lane within the row (texel x-major
+// index), pbw: row blocks = ceil((16 + max(nrandom, 0)) / 16) (kernel-uniform),
+// bkey: PRNG key of the block.  Lane 0 of the row gets the block in o.
+template <int CD>
+S2TC_D void encode_block_normal(uint32_t tex, int t, int w, int h, int dxt, int nrandom, int refine, uint32_t bkey,
+                                int pbw, const GroupSmem<CD> &sm, BlockOut &o)
+{
+	typedef Dist<CD> D;
+	typedef typename D::F F;
+	const int lane = (int)(threadIdx.x & 63);
+	const int gshift = lane & 48;
+	const int x = t >> 2, y = t & 3;
+	const bool valid = x < w && y < h;
+	const int alpha = texel_a(tex);
+	const bool use = valid && !(dxt == DXT1 && alpha == 0);
+	const uint32_t v = texel565(tex);
+
+	// gather (:940-959)
+	const uint32_t mask = (uint32_t)(__ballot(use) >> gshift) & 0xffffu;
+	const int pos = __builtin_popcount(mask & ((1u << t) - 1u));
+	int n = __builtin_popcount(mask);
+	if (use) sm.cand[pos] = v | ((uint32_t)alpha << 16);
+	const bool none = n == 0;
+	if (none) n = 1;
+	if (none && t == 0) sm.cand[0] = 0u;
+	__syncthreads();
+	int m = n;
+	if (nrandom > 0) {  // (:962-993)
+		int r = c_r(v), g = c_g(v), b = c_b(v);
+		int minr = row_min(use ? r : 255), ming = row_min(use ? g : 255), minb = row_min(use ? b : 255);
+		int maxr = row_max(use ? r : 0), maxg = row_max(use ? g : 0), maxb = row_max(use ? b : 0);
+		int mina = row_min(use ? alpha : 255), maxa = row_max(use ? alpha : 0);
+		if (none) minr = ming = minb = maxr = maxg = maxb = mina = maxa = 0;
+		const int lr = maxr - minr + 1, lg = maxg - ming + 1, lb = maxb - minb + 1, la = maxa - mina + 1;
+		const int dpc = dxt == DXT5 ? 4 : 3;
+		for (int c = t; c < nrandom; c += 16) {
+			const uint32_t k = (uint32_t)(c * dpc);
+			const uint32_t cr = (uint32_t)(minr + (int)((uint32_t)s2tc_prng_draw(bkey, k) % (uint32_t)lr));
+			const uint32_t cg = (uint32_t)(ming + (int)((uint32_t)s2tc_prng_draw(bkey, k + 1) % (uint32_t)lg));
+			const uint32_t cb = (uint32_t)(minb + (int)((uint32_t)s2tc_prng_draw(bkey, k + 2) % (uint32_t)lb));
+			uint32_t ca = 0;
+			if (dxt == DXT5) ca = (uint32_t)(mina + (int)((uint32_t)s2tc_prng_draw(bkey, k + 3) % (uint32_t)la));
+			sm.cand[n + c] = (cr << 11) | (cg << 5) | cb | (ca << 16);
+		}
+		m = n + nrandom;
+	} else if (n == 1) {  // "hack for last miplevel" (:996-1001); ca[1] pinned to 0
+		if (t == 0) sm.cand[1] = sm.cand[0] & 0xffffu;
+		n = m = 2;
+	}
+	__syncthreads();
+
+	// features of every candidate
+	for (int i = t; i < 16 * pbw; i += 16)
+		if (i < m) sm.feat[i] = D::feat(sm.cand[i] & 0xffffu);
+	__syncthreads();
+
+	// dists[i][k] rows (:373-392): square part dist(c[min], c[max]), box part dist(c[i], c[k])
+	for (int p = 0; p < pbw; ++p) {
+		const int i = 16 * p + t;
+		const bool irow = i < m;
+		F fi = irow ? sm.feat[i] : sm.feat[0];
+		int dv[16];
+#pragma unroll
+		for (int k = 0; k < 16; ++k) {
+			F fk = sm.feat[k];
+			int d;
+			if (IsSymmetric<CD>::value) d = D::d(fi, fk);
+			else d = (i < n && k < i) ? D::d(fk, fi) : D::d(fi, fk);
+			dv[k] = (irow && k < n) ? d : 0;
+		}
+		int4 *rp = (int4 *)(sm.rows + i * S2TC_RS);
+#pragma unroll
+		for (int k = 0; k < 4; ++k) rp[k] = make_int4(dv[4 * k], dv[4 * k + 1], dv[4 * k + 2], dv[4 * k + 3]);
+	}
+	__syncthreads();
+	const uint32_t cbest = reduce_rows<!IsSymmetric<CD>::value>(sm.rows, t, m, pbw);
+	const int ci = min((int)(cbest >> 10), m - 1), cj = min((int)(cbest & 1023u), m - 1);
+	uint32_t c0 = sm.cand[ci] & 0xffffu, c1 = sm.cand[cj] & 0xffffu;
+	int a0 = 0, a1 = 0;
+	if (dxt == DXT5) {  // reduce_colors_inplace_2fixpoints (:1006, :415-478)
+		__syncthreads();
+		for (int p = 0; p < pbw; ++p) {
+			const int i = 16 * p + t;
+			const bool irow = i < m;
+			const int ai = irow ? (int)(sm.cand[i] >> 16) : 0;
+			int dv[16];
+#pragma unroll
+			for (int k = 0; k < 16; ++k) {
+				const int ak = (int)(sm.cand[k] >> 16);
+				const int f0 = adist(0, ak), f1 = adist(255, ak);
+				const int fk = f1 < f0 ? f1 : f0;
+				const int d = adist(ai, ak);
+				dv[k] = (irow && k < n) ? (d < fk ? d : fk) : 0;
+			}
+			int4 *rp = (int4 *)(sm.rows + i * S2TC_RS);
+#pragma unroll
+			for (int k = 0; k < 4; ++k) rp[k] = make_int4(dv[4 * k], dv[4 * k + 1], dv[4 * k + 2], dv[4 * k + 3]);
+		}
+		__syncthreads();
+		const uint32_t abest = reduce_rows<false>(sm.rows, t, m, pbw);
+		const int ai = min((int)(abest >> 10), m - 1), aj = min((int)(abest & 1023u), m - 1);
+		a0 = (int)(sm.cand[ai] >> 16);
+		a1 = (int)(sm.cand[aj] >> 16);
+	}
+	__syncthreads();
+
+	// equal colors are BAD (:1009-1027)
+	c1 = fix_equal565(c0, c1);
+	if (dxt == DXT5) a1 = fix_equal_a(a0, a1);
+
+	const F f = D::feat(v);
+	const int slot = y * 4 + x;
+	uint32_t cidx = (dxt == DXT1) ? color_refine_group<CD, true>(tex, valid, f, refine, c0, c1)
+	                              : color_refine_group<CD, false>(tex, valid, f, refine, c0, c1);
+	const uint32_t cbits = row_or(cidx << (2 * slot));
+	uint32_t w0 = 0, w1 = 0;
+	if (dxt == DXT3) {
+		const uint64_t ab = valid ? (uint64_t)alpha << (4 * slot) : 0;
+		w0 = row_or((uint32_t)ab);
+		w1 = row_or((uint32_t)(ab >> 32));
+	} else if (dxt == DXT5) {
+		const uint32_t aidx = alpha_refine_group(alpha, valid, refine, a0, a1);
+		const uint64_t ab = (uint64_t)aidx << (3 * slot);
+		const uint32_t lo = row_or((uint32_t)ab), hi = row_or((uint32_t)(ab >> 32));
+		const uint64_t bits = ((uint64_t)hi << 32) | lo;
+		const uint64_t word = (uint64_t)(a0 & 255) | ((uint64_t)(a1 & 255) << 8) | (bits << 16);
+		w0 = (uint32_t)word;
+		w1 = (uint32_t)(word >> 32);
+	}
+	if (dxt == DXT1) {
+		o.w[0] = c0 | (c1 << 16);
+		o.w[1] = cbits;
+		o.w[2] = o.w[3] = 0;
+	} else {
+		o.w[0] = w0;
+		o.w[1] = w1;
+		o.w[2] = c0 | (c1 << 16);
+		o.w[3] = cbits;
+	}
+}
+
+}  // namespace s2tc

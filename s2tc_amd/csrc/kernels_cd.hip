@@ -1,0 +1,135 @@
+// kernels_cd.hip - encoder kernels for one colour distance mode.
+//
+// Compiled once per mode with -DS2TC_CD=<ColorDistMode value>; each object
+// exports s2tc_kernels_cd<N>() returning its kernel table.  Splitting by mode
+// keeps each translation unit small and lets the build compile them in
+// parallel.
+#include "encode_normal.h"
+#include "kernel_table.h"
+
+#ifndef S2TC_CD
+#error "S2TC_CD must be defined"
+#endif
+
+namespace s2tc {
+
+// quantize one RGBA8 texel with DITHER_NONE (s2tc_algorithm.cpp:1269-1306)
+template <int DXT> S2TC_D uint32_t quant_none(uint32_t p)
+{
+	uint32_t r = (p & 0xffu) >> 3, g = ((p >> 8) & 0xffu) >> 2, b = ((p >> 16) & 0xffu) >> 3, a = p >> 24;
+	a = DXT == DXT1 ? a >> 7 : DXT == DXT3 ? a >> 4 : a;
+	return r | (g << 8) | (b << 16) | (a << 24);
+}
+S2TC_D uint32_t quant_none_rt(uint32_t p, int dxt)
+{
+	return dxt == DXT1 ? quant_none<DXT1>(p) : dxt == DXT3 ? quant_none<DXT3>(p) : quant_none<DXT5>(p);
+}
+
+// Load the 4x4 block (bx, by) into tx[x*4+y]; texels outside w x h are 0.
+template <int DXT, bool RAW>
+S2TC_D void load_block(const uint32_t *__restrict__ src, int width, int bx, int by, int w, int h, bool vec,
+                       uint32_t (&tx)[16])
+{
+#pragma unroll
+	for (int k = 0; k < 16; ++k) tx[k] = 0;
+#pragma unroll
+	for (int y = 0; y < 4; ++y) {
+		if (y < h) {
+			const uint32_t *row = src + (size_t)(4 * by + y) * (size_t)width + 4 * bx;
+			if (vec && w == 4) {
+				const uint4 q = *(const uint4 *)row;
+				tx[y] = q.x;
+				tx[4 + y] = q.y;
+				tx[8 + y] = q.z;
+				tx[12 + y] = q.w;
+			} else {
+#pragma unroll
+				for (int x = 0; x < 4; ++x)
+					if (x < w) tx[x * 4 + y] = row[x];
+			}
+		}
+	}
+	if (RAW) {
+#pragma unroll
+		for (int k = 0; k < 16; ++k) tx[k] = quant_none<DXT>(tx[k]);
+	}
+}
+
+template <int DXT> S2TC_D void store_block(uint8_t *dst, size_t b, const BlockOut &o)
+{
+	if (DXT == DXT1)
+		*(uint2 *)(dst + b * 8) = make_uint2(o.w[0], o.w[1]);
+	else
+		*(uint4 *)(dst + b * 16) = make_uint4(o.w[0], o.w[1], o.w[2], o.w[3]);
+}
+
+// Quick mode over one image: one lane per block, blocks in raster order.
+template <int CD, int DXT, bool RAW>
+__global__ __launch_bounds__(256) void k_fast(const uint32_t *__restrict__ src, int width, int height,
+                                              uint8_t *__restrict__ dst, int refine, int vec)
+{
+	const int bw = (width + 3) >> 2, bh = (height + 3) >> 2;
+	const int b = (int)(blockIdx.x * 256 + threadIdx.x);
+	if (b >= bw * bh) return;
+	const int by = b / bw, bx = b - by * bw;
+	const int w = min(4, width - 4 * bx), h = min(4, height - 4 * by);
+	uint32_t tx[16];
+	load_block<DXT, RAW>(src, width, bx, by, w, h, vec != 0, tx);
+	const BlockOut o = encode_block_fast<CD, DXT>(tx, w, h, refine);
+	store_block<DXT>(dst, (size_t)b, o);
+}
+
+// MODE_NORMAL over one image: 16 lanes per block, 4 blocks per 64-lane workgroup.
+template <int CD, bool RAW>
+__global__ __launch_bounds__(64) void k_normal(const uint32_t *__restrict__ src, int width, int height,
+                                               uint8_t *__restrict__ dst, int dxt, int nrandom, int refine,
+                                               unsigned long long seed, int pbw)
+{
+	extern __shared__ __align__(16) uint8_t smem[];
+	const int lane = (int)threadIdx.x;
+	const int g = lane >> 4, t = lane & 15;
+	const int bw = (width + 3) >> 2, bh = (height + 3) >> 2, nb = bw * bh;
+	int b = (int)blockIdx.x * 4 + g;
+	const bool have = b < nb;
+	if (!have) b = nb - 1;
+	const int by = b / bw, bx = b - by * bw;
+	const int w = min(4, width - 4 * bx), h = min(4, height - 4 * by);
+	const int x = t >> 2, y = t & 3;
+	uint32_t tex = 0;
+	if (x < w && y < h) {
+		tex = src[(size_t)(4 * by + y) * (size_t)width + 4 * bx + x];
+		if (RAW) tex = quant_none_rt(tex, dxt);
+	}
+	const uint32_t bkey = s2tc_prng_block_key((uint64_t)seed, (uint32_t)b);
+	const GroupSmem<S2TC_CD> sm = group_smem<S2TC_CD>(smem, g, 16 * pbw);
+	BlockOut o;
+	encode_block_normal<S2TC_CD>(tex, t, w, h, dxt, nrandom, refine, bkey, pbw, sm, o);
+	if (have && t == 0) {
+		if (dxt == DXT1) store_block<DXT1>(dst, (size_t)b, o);
+		else store_block<DXT5>(dst, (size_t)b, o);
+	}
+}
+
+}  // namespace s2tc
+
+using namespace s2tc;
+
+#define S2TC_CAT2(a, b) a##b
+#define S2TC_CAT(a, b) S2TC_CAT2(a, b)
+
+extern "C" void S2TC_CAT(s2tc_kernels_cd, S2TC_CD)(KernelTable *kt)
+{
+#if S2TC_CD != 7  /* NORMALMAP never runs in quick mode (s2tc_algorithm.cpp:1131-1134) */
+	kt->fast[DXT1][0] = (const void *)k_fast<S2TC_CD, DXT1, false>;
+	kt->fast[DXT3][0] = (const void *)k_fast<S2TC_CD, DXT3, false>;
+	kt->fast[DXT5][0] = (const void *)k_fast<S2TC_CD, DXT5, false>;
+	kt->fast[DXT1][1] = (const void *)k_fast<S2TC_CD, DXT1, true>;
+	kt->fast[DXT3][1] = (const void *)k_fast<S2TC_CD, DXT3, true>;
+	kt->fast[DXT5][1] = (const void *)k_fast<S2TC_CD, DXT5, true>;
+#else
+	for (int d = 0; d < 3; ++d) kt->fast[d][0] = kt->fast[d][1] = nullptr;
+#endif
+	kt->normal[0] = (const void *)k_normal<S2TC_CD, false>;
+	kt->normal[1] = (const void *)k_normal<S2TC_CD, true>;
+	kt->normal_smem_per_cand = (int)(4 * (sizeof(uint32_t) + sizeof(Dist<S2TC_CD>::F) + S2TC_RS * sizeof(int)));
+}

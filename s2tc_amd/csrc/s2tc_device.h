@@ -1,0 +1,223 @@
+// s2tc_device.h - device-side building blocks of the S2TC encoder (gfx950).
+//
+// Colours are handled as RGB565 words v = r << 11 | g << 5 | b.  The
+// reference's color_t order (lexicographic r, g, b; s2tc_algorithm.cpp:70-81)
+// is then the numeric order of v, its ++/-- lattice steps (:82-131) are
+// v +/- 1 modulo 65536, and its packed endpoint bytes (:1046-1049) are the
+// little-endian bytes of v.
+//
+// Distances are evaluated on per-colour "features" so that work which the
+// reference repeats on every call (squares for SRGB, the sqrt luminance of
+// SRGB_MIXED, the normalisation of NORMALMAP) is done once per colour; the
+// remaining arithmetic is the reference's, in its order, with int32
+// wrap-around where the reference overflows (SRGB).  Float code relies on
+// -ffp-contract=off and on HIP's correctly rounded fp32 '/' and sqrtf.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "s2tc_prng.h"
+
+namespace s2tc {
+
+enum { DXT1 = 0, DXT3 = 1, DXT5 = 2 };
+enum { REFINE_NEVER = 0, REFINE_ALWAYS = 1, REFINE_LOOP = 2 };
+enum { CD_RGB = 0, CD_YUV = 1, CD_SRGB = 2, CD_SRGB_MIXED = 3, CD_AVG = 4, CD_WAVG = 5, CD_W0AVG = 6,
+       CD_NORMALMAP = 7 };
+
+#define S2TC_D __device__ __forceinline__
+
+S2TC_D int c_r(uint32_t v) { return (int)(v >> 11) & 31; }
+S2TC_D int c_g(uint32_t v) { return (int)(v >> 5) & 63; }
+S2TC_D int c_b(uint32_t v) { return (int)v & 31; }
+// quantized texel (bytes r5, g6, b5, a) -> RGB565 word
+S2TC_D uint32_t texel565(uint32_t t)
+{
+	return ((t & 0x1fu) << 11) | (((t >> 8) & 0x3fu) << 5) | ((t >> 16) & 0x1fu);
+}
+S2TC_D int texel_a(uint32_t t) { return (int)(t >> 24); }
+
+// SHRR(a,n) = ((a) + (1 << ((n)-1))) >> (n) with wrap-around (s2tc_algorithm.cpp:215)
+S2TC_D int shrr(int a, int n) { return (int)((uint32_t)a + (1u << (n - 1))) >> n; }
+S2TC_D int wmul(int a, int b) { return (int)((uint32_t)a * (uint32_t)b); }
+S2TC_D int wadd(int a, int b) { return (int)((uint32_t)a + (uint32_t)b); }
+
+struct FeatI { int x, y, z; };
+struct FeatF { float x, y, z; };
+
+template <int CD> struct Dist;
+
+// AVG: 4dr^2 + dg^2 + 4db^2  (:217-223)
+template <> struct Dist<CD_AVG> {
+	typedef FeatI F;
+	static S2TC_D F feat(uint32_t v) { return F{c_r(v), c_g(v), c_b(v)}; }
+	static S2TC_D int d(const F &a, const F &b)
+	{
+		int dr = a.x - b.x, dg = a.y - b.y, db = a.z - b.z;
+		return ((dr * dr) << 2) + dg * dg + ((db * db) << 2);
+	}
+};
+// W0AVG: dr^2 + dg^2 + db^2  (:225-232)
+template <> struct Dist<CD_W0AVG> {
+	typedef FeatI F;
+	static S2TC_D F feat(uint32_t v) { return F{c_r(v), c_g(v), c_b(v)}; }
+	static S2TC_D int d(const F &a, const F &b)
+	{
+		int dr = a.x - b.x, dg = a.y - b.y, db = a.z - b.z;
+		return dr * dr + dg * dg + db * db;
+	}
+};
+// WAVG: 4dr^2 + 4dg^2 + db^2  (:234-241)
+template <> struct Dist<CD_WAVG> {
+	typedef FeatI F;
+	static S2TC_D F feat(uint32_t v) { return F{c_r(v), c_g(v), c_b(v)}; }
+	static S2TC_D int d(const F &a, const F &b)
+	{
+		int dr = a.x - b.x, dg = a.y - b.y, db = a.z - b.z;
+		return ((dr * dr) << 2) + ((dg * dg) << 2) + (db * db);
+	}
+};
+// YUV / RGB: weighted Y'UV difference (:243-267); no overflow is possible
+// (|y| <= 6259*31, so every term stays far below 2^31).
+template <int KR, int KG, int KB> struct DistYuvLike {
+	typedef FeatI F;
+	static S2TC_D F feat(uint32_t v) { return F{c_r(v), c_g(v), c_b(v)}; }
+	static S2TC_D int d(const F &a, const F &b)
+	{
+		int dr = a.x - b.x, dg = a.y - b.y, db = a.z - b.z;
+		int y = dr * KR + dg * KG + db * KB;
+		int u = dr * 202 - y;
+		int v = db * 202 - y;
+		return ((y * y) << 1) + shrr(u * u, 3) + shrr(v * v, 4);
+	}
+};
+template <> struct Dist<CD_YUV> : DistYuvLike<60, 59, 22> {};
+template <> struct Dist<CD_RGB> : DistYuvLike<42, 72, 14> {};
+// SRGB (:269-283): differences of squared components; SHRR(y,3)*SHRR(y,4)
+// overflows int32 for large differences and the reference wraps, so the
+// products go through uint32.  Not symmetric in its arguments.
+template <> struct Dist<CD_SRGB> {
+	typedef FeatI F;
+	static S2TC_D F feat(uint32_t v)
+	{
+		int r = c_r(v), g = c_g(v), b = c_b(v);
+		return F{r * r, g * g, b * b};
+	}
+	static S2TC_D int d(const F &a, const F &b)
+	{
+		int dr = a.x - b.x, dg = a.y - b.y, db = a.z - b.z;
+		int y = dr * 84 + dg * 72 + db * 28;
+		int u = dr * 409 - y;
+		int v = db * 409 - y;
+		int sy = wmul(shrr(y, 3), shrr(y, 4));
+		int su = wmul(shrr(u, 3), shrr(u, 4));
+		int sv = wmul(shrr(v, 3), shrr(v, 4));
+		return wadd(wadd(shrr(sy, 4), shrr(su, 8)), shrr(sv, 9));
+	}
+};
+// srgb_get_y (:285-296): (int)(sqrtf(37*(84r^2+72g^2+28b^2)) + 0.5f); the
+// argument is below 2^24 so the int->float conversion is exact.
+S2TC_D int srgb_get_y(int r, int g, int b)
+{
+	int y = 37 * (r * r * 84 + g * g * 72 + b * b * 28);
+	return (int)(sqrtf((float)y) + 0.5f);
+}
+// SRGB_MIXED (:298-315): feature (Y, 191r - Y, 191b - Y).
+template <> struct Dist<CD_SRGB_MIXED> {
+	typedef FeatI F;
+	static S2TC_D F feat(uint32_t v)
+	{
+		int r = c_r(v), g = c_g(v), b = c_b(v);
+		int y = srgb_get_y(r, g, b);
+		return F{y, r * 191 - y, b * 191 - y};
+	}
+	static S2TC_D int d(const F &a, const F &b)
+	{
+		int y = a.x - b.x, u = a.y - b.y, v = a.z - b.z;
+		return ((y * y) << 3) + shrr(u * u, 1) + shrr(v * v, 2);
+	}
+};
+// NORMALMAP (:317-354): feature = the normalised vector, computed with the
+// reference's operation order; distance = (int)(100000 * sum (cb-ca)^2).
+template <> struct Dist<CD_NORMALMAP> {
+	typedef FeatF F;
+	static S2TC_D F feat(uint32_t v)
+	{
+		float x = (float)c_r(v) / 31.0f * 2.0f - 1.0f;
+		float y = (float)c_g(v) / 63.0f * 2.0f - 1.0f;
+		float z = (float)c_b(v) / 31.0f * 2.0f - 1.0f;
+		float n = x * x + y * y + z * z;
+		if (n > 0) {
+			n = 1.0f / sqrtf(n);
+			x *= n;
+			y *= n;
+			z *= n;
+		}
+		return F{x, y, z};
+	}
+	static S2TC_D int d(const F &a, const F &b)
+	{
+		float dx = b.x - a.x, dy = b.y - a.y, dz = b.z - a.z;
+		return (int)(100000.0f * (dx * dx + dy * dy + dz * dz));
+	}
+};
+
+template <int CD> struct IsSymmetric { static const bool value = true; };
+template <> struct IsSymmetric<CD_SRGB> { static const bool value = false; };
+
+// alpha_dist (:358-361)
+S2TC_D int adist(int a, int b) { return (a - b) * (a - b); }
+
+// Unsigned x / d for 0 <= x < 2^20 and 1 <= d <= 64: float reciprocal
+// estimate, then one exact integer correction (the estimate is never above
+// the true quotient by a whole unit for these ranges).
+S2TC_D uint32_t udiv_small(uint32_t x, uint32_t d)
+{
+	float fq = (float)x * __builtin_amdgcn_rcpf((float)d);
+	uint32_t q = (uint32_t)fq;
+	int r = (int)(x - q * d);
+	q = r < 0 ? q - 1 : q;
+	r = r < 0 ? r + (int)d : r;
+	q += (uint32_t)r >= d ? 1u : 0u;
+	return q;
+}
+// ((S << 1) + n) / (n << 1): round-half-up mean (:547-549)
+S2TC_D int rmean(int s, int n) { return (int)udiv_small((uint32_t)((s << 1) + n), (uint32_t)(n << 1)); }
+
+// 2-bit colour index word fix-ups (:802-804, :810-812)
+S2TC_D uint32_t keep_ones(uint32_t bits)
+{
+	uint32_t lo = bits & 0x55555555u, hi = (bits >> 1) & 0x55555555u;
+	return lo & ~hi;
+}
+S2TC_D uint32_t swap_remap2(uint32_t bits) { return bits ^ (~(bits >> 1) & 0x55555555u); }
+// 3-bit alpha index word fix-ups (:679-684, :690-704), 16 slots in 48 bits
+#define S2TC_M3 0x249249249249ull
+S2TC_D uint64_t keep_not_one3(uint64_t bits)
+{
+	uint64_t b0 = bits & S2TC_M3, b1 = (bits >> 1) & S2TC_M3, b2 = (bits >> 2) & S2TC_M3;
+	uint64_t one = b0 & ~b1 & ~b2;
+	return bits & ~one;
+}
+S2TC_D uint64_t swap_remap3(uint64_t bits)
+{
+	uint64_t b1 = (bits >> 1) & S2TC_M3, b2 = (bits >> 2) & S2TC_M3;
+	uint64_t flip_all = b1 ^ b2;        // 2..5 -> 7 - v
+	uint64_t low = ~(b1 | b2) & S2TC_M3; // 0 <-> 1
+	return bits ^ (flip_all * 7u) ^ low;
+}
+
+// equal endpoints fix (:1009-1016): c1 := c1 -/+ 1 on the 565 lattice
+S2TC_D uint32_t fix_equal565(uint32_t v0, uint32_t v1)
+{
+	if (v0 != v1) return v1;
+	return v0 == 0xffffu ? v1 - 1u : ((v1 + 1u) & 0xffffu);
+}
+S2TC_D int fix_equal_a(int a0, int a1)
+{
+	if (a0 != a1) return a1;
+	return a0 == 255 ? a1 - 1 : a1 + 1;
+}
+
+}  // namespace s2tc

@@ -1,0 +1,158 @@
+// s2tc_host.cpp - host side of the GPU encoder: kernel selection, launch
+// geometry, device memory plumbing and error reporting.
+//
+// The dispatch rules mirror s2tc_encode_block_func (s2tc_algorithm.cpp
+// :1110-1194): out-of-range refine -> ALWAYS, dxt -> DXT5, distance -> WAVG;
+// quick mode (one lane per block) iff nrandom < 0 and the distance is not
+// NORMALMAP; everything else runs the 16-lanes-per-block MODE_NORMAL kernel.
+#include "s2tc_internal.h"
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+
+namespace s2tc {
+
+static thread_local char g_err[512];
+
+int set_error(int code, const char *fmt, ...)
+{
+	va_list ap;
+	va_start(ap, fmt);
+	vsnprintf(g_err, sizeof g_err, fmt, ap);
+	va_end(ap);
+	return code;
+}
+
+int hip_check(hipError_t e, const char *what)
+{
+	if (e == hipSuccess) return 0;
+	return set_error(S2TC_E_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+static KernelTable g_tables[8];
+static std::once_flag g_tables_once;
+
+const KernelTable &kernel_table(int cd)
+{
+	std::call_once(g_tables_once, [] {
+		s2tc_kernels_cd0(&g_tables[0]);
+		s2tc_kernels_cd1(&g_tables[1]);
+		s2tc_kernels_cd2(&g_tables[2]);
+		s2tc_kernels_cd3(&g_tables[3]);
+		s2tc_kernels_cd4(&g_tables[4]);
+		s2tc_kernels_cd5(&g_tables[5]);
+		s2tc_kernels_cd6(&g_tables[6]);
+		s2tc_kernels_cd7(&g_tables[7]);
+	});
+	return g_tables[cd];
+}
+
+void normalize_modes(int &dxt, int &cd, int &refine)
+{
+	if (refine != REFINE_NEVER && refine != REFINE_LOOP) refine = REFINE_ALWAYS;
+	if (dxt != DXT1 && dxt != DXT3) dxt = DXT5;
+	if (cd < CD_RGB || cd > CD_NORMALMAP) cd = CD_WAVG;
+}
+
+bool is_fast(int cd, int nrandom) { return cd != CD_NORMALMAP && nrandom < 0; }
+
+int normal_pbw(int nrandom) { return (16 + (nrandom > 0 ? nrandom : 0) + 15) / 16; }
+
+int ensure_device()
+{
+	static int ok = -1;
+	if (ok < 0) {
+		int n = 0;
+		hipError_t e = hipGetDeviceCount(&n);
+		ok = (e == hipSuccess && n > 0) ? 1 : 0;
+	}
+	if (!ok) return set_error(S2TC_E_NOGPU, "no HIP device available");
+	return 0;
+}
+
+int launch_encode(const EncodeArgs &a, hipStream_t stream)
+{
+	int dxt = a.dxt, cd = a.cd, refine = a.refine;
+	normalize_modes(dxt, cd, refine);
+	if (a.width <= 0 || a.height <= 0) return set_error(S2TC_E_ARG, "bad image size %dx%d", a.width, a.height);
+	if (a.src_kind != S2TC_SRC_QUANTIZED && a.src_kind != S2TC_SRC_RGBA8)
+		return set_error(S2TC_E_ARG, "bad source kind %d", a.src_kind);
+	if (a.nrandom > S2TC_MAX_NRANDOM)
+		return set_error(S2TC_E_RANGE, "nrandom %d exceeds the GPU limit %d", a.nrandom, S2TC_MAX_NRANDOM);
+	if (int r = ensure_device()) return r;
+	const KernelTable &kt = kernel_table(cd);
+	const int raw = a.src_kind == S2TC_SRC_RGBA8 ? 1 : 0;
+	const long long nb = (long long)((a.width + 3) / 4) * ((a.height + 3) / 4);
+	if (nb > (1LL << 30)) return set_error(S2TC_E_RANGE, "image too large");
+	const void *src = a.src;
+	void *dst = a.dst;
+	int width = a.width, height = a.height;
+	if (is_fast(cd, a.nrandom)) {
+		if (!kt.fast[dxt][raw]) return set_error(S2TC_E_ARG, "no quick-mode kernel for distance %d", cd);
+		int vec = ((width & 3) == 0 && (((uintptr_t)src) & 15) == 0) ? 1 : 0;
+		void *args[] = {(void *)&src, (void *)&width, (void *)&height, (void *)&dst, (void *)&refine, (void *)&vec};
+		dim3 grid((unsigned)((nb + 255) / 256)), block(256);
+		return hip_check(hipLaunchKernel(kt.fast[dxt][raw], grid, block, args, 0, stream), "launch k_fast");
+	}
+	int nrandom = a.nrandom;
+	int pbw = normal_pbw(nrandom);
+	unsigned long long seed = a.seed;
+	size_t smem = (size_t)kt.normal_smem_per_cand * 16 * pbw;
+	void *args[] = {(void *)&src, (void *)&width, (void *)&height, (void *)&dst, (void *)&dxt,
+	                (void *)&nrandom, (void *)&refine, (void *)&seed, (void *)&pbw};
+	dim3 grid((unsigned)((nb + 3) / 4)), block(64);
+	return hip_check(hipLaunchKernel(kt.normal[raw], grid, block, args, smem, stream), "launch k_normal");
+}
+
+}  // namespace s2tc
+
+using namespace s2tc;
+
+extern "C" {
+
+const char *s2tc_amd_last_error(void) { return g_err; }
+
+int s2tc_amd_device_count(void)
+{
+	int n = 0;
+	if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+	return n;
+}
+
+int s2tc_amd_malloc(void **dptr, size_t bytes)
+{
+	if (!dptr) return set_error(S2TC_E_ARG, "null pointer");
+	if (int r = ensure_device()) return r;
+	return hip_check(hipMalloc(dptr, bytes ? bytes : 1), "hipMalloc");
+}
+
+int s2tc_amd_free(void *dptr) { return hip_check(hipFree(dptr), "hipFree"); }
+
+int s2tc_amd_memcpy_h2d(void *dst, const void *src, size_t bytes)
+{
+	return hip_check(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice), "hipMemcpy H2D");
+}
+
+int s2tc_amd_memcpy_d2h(void *dst, const void *src, size_t bytes)
+{
+	return hip_check(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+}
+
+int s2tc_amd_memset(void *dst, int value, size_t bytes)
+{
+	return hip_check(hipMemset(dst, value, bytes), "hipMemset");
+}
+
+int s2tc_amd_synchronize(void) { return hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize"); }
+
+int s2tc_amd_encode_image(const void *d_src, int src_kind, int width, int height, void *d_dst, int dxt, int cd,
+                          int nrandom, int refine, uint64_t seed, void *stream)
+{
+	if (!d_src || !d_dst) return set_error(S2TC_E_ARG, "null device pointer");
+	EncodeArgs a{d_src, d_dst, src_kind, width, height, dxt, cd, nrandom, refine, seed};
+	return launch_encode(a, (hipStream_t)stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,43 @@
+// s2tc_internal.h - shared declarations of the host-side library code.
+#pragma once
+
+#ifndef __HIP_PLATFORM_AMD__
+#define __HIP_PLATFORM_AMD__ 1
+#endif
+#include <hip/hip_runtime_api.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "kernel_table.h"
+#include "s2tc_amd.h"
+
+namespace s2tc {
+
+enum { DXT1 = 0, DXT3 = 1, DXT5 = 2 };
+enum { REFINE_NEVER = 0, REFINE_ALWAYS = 1, REFINE_LOOP = 2 };
+enum { CD_RGB = 0, CD_YUV = 1, CD_SRGB = 2, CD_SRGB_MIXED = 3, CD_AVG = 4, CD_WAVG = 5, CD_W0AVG = 6,
+       CD_NORMALMAP = 7 };
+enum { DITHER_NONE = 0, DITHER_SIMPLE = 1, DITHER_FLOYDSTEINBERG = 2 };
+
+struct EncodeArgs {
+	const void *src;
+	void *dst;
+	int src_kind;
+	int width, height;
+	int dxt, cd, nrandom, refine;
+	uint64_t seed;
+};
+
+int set_error(int code, const char *fmt, ...);
+int hip_check(hipError_t e, const char *what);
+int ensure_device();
+const KernelTable &kernel_table(int cd);
+void normalize_modes(int &dxt, int &cd, int &refine);
+bool is_fast(int cd, int nrandom);
+int normal_pbw(int nrandom);
+int launch_encode(const EncodeArgs &a, hipStream_t stream);
+
+inline int block_bytes(int dxt) { return dxt == DXT1 ? 8 : 16; }
+
+}  // namespace s2tc

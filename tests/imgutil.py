@@ -1,0 +1,72 @@
+"""Seeded synthetic test inputs (quantized texels and RGBA8 images)."""
+import numpy as np
+
+ALPHABITS = {0: 1, 1: 4, 2: 8}
+
+
+def random_quantized(rng, w, h, dxt, kind="random"):
+    """Quantized texels (r5, g6, b5, a) as rgb565_image would produce them."""
+    q = np.zeros((h, w, 4), np.uint8)
+    ab = ALPHABITS[dxt]
+    if kind == "random":
+        q[..., 0] = rng.integers(0, 32, (h, w))
+        q[..., 1] = rng.integers(0, 64, (h, w))
+        q[..., 2] = rng.integers(0, 32, (h, w))
+        q[..., 3] = rng.integers(0, 1 << ab, (h, w))
+    elif kind == "smooth":
+        base = rng.integers(0, 26, 3)
+        q[..., 0] = base[0] + rng.integers(0, 6, (h, w))
+        q[..., 1] = 2 * base[1] + rng.integers(0, 12, (h, w))
+        q[..., 2] = base[2] + rng.integers(0, 6, (h, w))
+        q[..., 3] = rng.integers(0, 1 << ab, (h, w))
+    elif kind == "speckle":
+        q[..., 0] = rng.integers(0, 32, (h, w))
+        q[..., 1] = rng.integers(0, 64, (h, w))
+        q[..., 2] = rng.integers(0, 32, (h, w))
+        q[..., 3] = np.where(rng.random((h, w)) < 0.5, 0, (1 << ab) - 1)
+    elif kind == "extreme":   # saturated colours: exercises the SRGB int32 wrap
+        q[..., 0] = rng.choice([0, 1, 30, 31], (h, w))
+        q[..., 1] = rng.choice([0, 1, 60, 63], (h, w))
+        q[..., 2] = rng.choice([0, 1, 30, 31], (h, w))
+        q[..., 3] = rng.choice([0, (1 << ab) - 1], (h, w))
+    else:
+        raise ValueError(kind)
+    return q.reshape(-1)
+
+
+def special_blocks(dxt):
+    """A 4-block-row image of hand-picked 4x4 blocks (returns q, w, h)."""
+    ab = (1 << ALPHABITS[dxt]) - 1
+    blocks = []
+    def blk(fn):
+        b = np.zeros((4, 4, 4), np.uint8)
+        for y in range(4):
+            for x in range(4):
+                b[y, x] = fn(x, y)
+        blocks.append(b)
+    blk(lambda x, y: (31, 63, 31, ab))                   # uniform white
+    blk(lambda x, y: (0, 0, 0, ab))                      # uniform black
+    blk(lambda x, y: (12, 40, 7, 0))                     # all transparent
+    blk(lambda x, y: (5, 10, 20, ab) if (x + y) & 1 else (25, 50, 2, ab))   # two colours
+    blk(lambda x, y: (x * 8, y * 16, (x + y) * 3, ab if (x * y) & 1 else 0))
+    blk(lambda x, y: (0, 60, 0, ab) if x < 2 else (30, 0, 30, ab))           # SRGB negative distance pair
+    blk(lambda x, y: (31, 63, 31, ab) if x == 0 and y == 0 else (31, 63, 30, ab))
+    blk(lambda x, y: (7, 7, 7, ab if x == 3 and y == 3 else 0))              # one opaque texel
+    blk(lambda x, y: (x * 10, 63 - y * 20, 31 - x * 10, (x * 60 + y * 15) % (ab + 1)))
+    blk(lambda x, y: (0, 0, 0, ab) if (x, y) == (2, 1) else (31, 63, 31, ab))
+    n = len(blocks)
+    img = np.zeros((4, 4 * n, 4), np.uint8)
+    for i, b in enumerate(blocks):
+        img[:, 4 * i:4 * i + 4] = b
+    return img.reshape(-1), 4 * n, 4
+
+
+def smooth_rgba8(rng, w, h):
+    """Natural-looking RGBA8 content (sinusoids + noise)."""
+    y, x = np.mgrid[0:h, 0:w].astype(np.float64)
+    ph = rng.random(4) * 6.28
+    img = np.empty((h, w, 4), np.uint8)
+    for c in range(4):
+        v = 128 + 100 * np.sin(x / 7.0 + ph[c]) * np.cos(y / 9.0 + ph[c]) + rng.integers(-20, 21, (h, w))
+        img[..., c] = np.clip(v, 0, 255).astype(np.uint8)
+    return img.reshape(-1)
