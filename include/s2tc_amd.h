@@ -21,6 +21,14 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#ifndef S2TC_API
+#if defined(__GNUC__)
+#define S2TC_API __attribute__((visibility("default")))
+#else
+#define S2TC_API
+#endif
+#endif
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -40,22 +48,22 @@ extern "C" {
 #define S2TC_SRC_QUANTIZED 0   /* rgb565_image output: r5 g6 b5 a, 4 bytes per texel */
 #define S2TC_SRC_RGBA8     1   /* 8-bit RGBA, quantized on load with DITHER_NONE */
 
-const char *s2tc_amd_last_error(void);
-int s2tc_amd_device_count(void);
+S2TC_API const char *s2tc_amd_last_error(void);
+S2TC_API int s2tc_amd_device_count(void);
 
 /* device memory and stream plumbing (HIP device 0 unless set otherwise) */
-int s2tc_amd_malloc(void **dptr, size_t bytes);
-int s2tc_amd_free(void *dptr);
-int s2tc_amd_memcpy_h2d(void *dst, const void *src, size_t bytes);
-int s2tc_amd_memcpy_d2h(void *dst, const void *src, size_t bytes);
-int s2tc_amd_memset(void *dst, int value, size_t bytes);
-int s2tc_amd_synchronize(void);
+S2TC_API int s2tc_amd_malloc(void **dptr, size_t bytes);
+S2TC_API int s2tc_amd_free(void *dptr);
+S2TC_API int s2tc_amd_memcpy_h2d(void *dst, const void *src, size_t bytes);
+S2TC_API int s2tc_amd_memcpy_d2h(void *dst, const void *src, size_t bytes);
+S2TC_API int s2tc_amd_memset(void *dst, int value, size_t bytes);
+S2TC_API int s2tc_amd_synchronize(void);
 
 /* Encode one image already resident on the device.  d_src: width*height
  * texels of 4 bytes (row stride = width); d_dst: ceil(w/4)*ceil(h/4) blocks
  * of 8 (DXT1) or 16 bytes in raster order.  Asynchronous on stream (a
  * hipStream_t, NULL = default stream). */
-int s2tc_amd_encode_image(const void *d_src, int src_kind, int width, int height, void *d_dst,
+S2TC_API int s2tc_amd_encode_image(const void *d_src, int src_kind, int width, int height, void *d_dst,
                           int dxt, int cd, int nrandom, int refine, uint64_t seed, void *stream);
 
 #ifdef __cplusplus

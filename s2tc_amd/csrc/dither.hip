@@ -16,8 +16,23 @@
 //   5. emit:         one thread per chunk re-runs its texels from the start
 //                    state and writes the quantized values.
 // The mod-255 alpha channel uses sums instead of maps.
+//
+// DITHER_FLOYDSTEINBERG (:1350-1412) diffuses a 12-bit error to the right
+// (7/16) and to the row below (3, 5 and the rest of the remainder).  Texel
+// (x, y) reads the accumulated error A_y[x+1] that row y-1 finalises when it
+// processes texel x+1, so rows form a wavefront with a skew of two texels:
+// thread r of one workgroup handles row y0 + r and at step t texel
+// x = t - 2r.  Each row keeps a window of its not yet final downward errors
+// in registers and hands A_{y+1}[x] to the next row through LDS (double
+// buffered by step parity); the last row of a band hands them to the first
+// row of the next band through a global carry row.  The reference reuses one
+// error buffer for the colour and the alpha pass and does not clear the
+// part the alpha pass reads for its first row: that row starts from the
+// red-channel errors the colour pass left there (see k_dither_fs).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include "s2tc_internal.h"
 
 namespace s2tc {
 
@@ -101,6 +116,19 @@ __global__ void k_dither_super_starts(const uint8_t *__restrict__ smaps, long lo
 	}
 }
 
+// 3'. as k_dither_super_starts, reading the super maps from global memory
+__global__ void k_dither_super_starts_global(const uint8_t *__restrict__ smaps, long long nsuper, DChan ch,
+                                             uint8_t *__restrict__ sstart)
+{
+	if (threadIdx.x != 0) return;
+	const int S = ch.nstates;
+	int s = ch.kind == 1 ? 127 : (S - 1) / 2;
+	for (long long sc = 0; sc < nsuper; ++sc) {
+		sstart[sc] = (uint8_t)s;
+		s = ch.kind == 1 ? (s + smaps[sc]) % 255 : smaps[sc * S + s];
+	}
+}
+
 // 4. start state of every chunk
 __global__ void k_dither_chunk_starts(const uint8_t *__restrict__ maps, long long nchunks, DChan ch,
                                       const uint8_t *__restrict__ sstart, long long nsuper,
@@ -161,6 +189,205 @@ __global__ void k_quantize_none(const uint8_t *__restrict__ src, int srccomps, l
 			out[p * 4 + 3] = (uint8_t)((1 << alphabits) - 1);
 		}
 	}
+}
+
+__device__ __forceinline__ int floyd_q(int src, int acc, int shift, int maxval, int &err)
+{
+	const int s = ((src << 4) | (src >> 4)) + acc;
+	int r = s >> (shift + 4);
+	r = r < 0 ? 0 : (r > maxval ? maxval : r);
+	err = s - r * 4095 / maxval;
+	return r;
+}
+__device__ __forceinline__ int floyd1_q(int src, int acc, int &err)
+{
+	const int s = ((src << 4) | (src >> 4)) + acc;
+	const int r = s >= 2048 ? 1 : 0;
+	err = s - (r ? 4095 : 0);
+	return r;
+}
+// e7, e3, e5, e1 of floyd/floyd1 (:1227-1233), C division (toward zero)
+__device__ __forceinline__ void fs_split(int err, int &e7, int &e3, int &e5, int &e1)
+{
+	e7 = (err * 7 + 8) / 16;
+	err -= e7;
+	e3 = (err * 3 + 4) / 9;
+	err -= e3;
+	e5 = (err * 5 + 3) / 6;
+	err -= e5;
+	e1 = err;
+}
+
+// ---- Floyd-Steinberg ----
+#define FS_R 256
+
+struct FsChan {
+	int off;    // byte offset of the channel in a source texel
+	int kind;   // 0: floyd(shift), 1: floyd1
+	int shift;
+};
+
+__device__ __forceinline__ int fs_load(const int *p)
+{
+	return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void fs_store(int *p, int v)
+{
+	__hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// One Floyd-Steinberg pass over NCH independent channels.  carry: NCH*(w+2)
+// ints of scratch.  init0 (w+2 ints, may be null) is A_0 of channel 0, the
+// errors its first row starts from (zero otherwise).  rleft (w+2 ints, may
+// be null) receives what the reference's buffer holds in the first w+2 ints
+// after the pass, for channel 0: the accumulated errors the last row read
+// (thisrow, when h-1 is even) or that row's downward errors (downrow, when
+// h-1 is odd).
+template <int NCH>
+__global__ __launch_bounds__(FS_R) void k_dither_fs(const uint8_t *__restrict__ src, int srccomps, int w, int h,
+                                                    FsChan c0, FsChan c1, FsChan c2, const int *__restrict__ init0,
+                                                    int *__restrict__ carry, int *__restrict__ rleft,
+                                                    uint8_t *__restrict__ out)
+{
+	__shared__ int slot[NCH][FS_R][2];
+	const FsChan chs[3] = {c0, c1, c2};
+	const int r = (int)threadIdx.x;
+	const int pw = w + 2;
+	for (int j = r; j < pw; j += FS_R) {
+		fs_store(&carry[j], init0 ? init0[j] : 0);
+		for (int c = 1; c < NCH; ++c) fs_store(&carry[c * pw + j], 0);
+	}
+	__syncthreads();
+	for (int y0 = 0; y0 < h; y0 += FS_R) {
+		const int rows = min(FS_R, h - y0);
+		const int y = y0 + r;
+		const bool active = r < rows;
+		int right[NCH], win1[NCH], win2[NCH];
+#pragma unroll
+		for (int c = 0; c < NCH; ++c) right[c] = win1[c] = win2[c] = 0;
+		const int steps = w + 2 * (rows - 1) + 1;
+		for (int t = 0; t < steps; ++t) {
+			const int x = t - 2 * r;
+			if (active && x >= 0 && x <= w) {
+#pragma unroll
+				for (int c = 0; c < NCH; ++c) {
+					const FsChan ch = chs[c];
+					int fin;
+					if (x < w) {
+						const int above = r == 0 ? fs_load(&carry[c * pw + x + 1]) : slot[c][r - 1][(t - 1) & 1];
+						const int acc = above + right[c];
+						const int sv = src[((size_t)y * w + x) * srccomps + ch.off];
+						int err, q;
+						q = ch.kind == 1 ? floyd1_q(sv, acc, err)
+						                 : floyd_q(sv, acc, ch.shift, (1 << (8 - ch.shift)) - 1, err);
+						out[((size_t)y * w + x) * 4 + ch.off] = (uint8_t)q;
+						int e7, e3, e5, e1;
+						fs_split(err, e7, e3, e5, e1);
+						right[c] = e7;
+						fin = win1[c] + e3;
+						win1[c] = win2[c] + e5;
+						win2[c] = e1;
+						if (c == 0 && rleft && y == h - 1 && ((h - 1) & 1) == 0) rleft[x + 1] = acc;
+					} else {
+						fin = win1[c];
+					}
+					if (r + 1 < rows) slot[c][r][t & 1] = fin;  // A_{y+1}[x] for row y+1
+					else fs_store(&carry[c * pw + x], fin);
+					if (c == 0 && rleft && y == h - 1 && ((h - 1) & 1) == 1) rleft[x] = fin;
+				}
+			}
+			__syncthreads();
+		}
+	}
+}
+
+static int launch_rgb565_fs(const uint8_t *src, int srccomps, int w, int h, int alphabits, uint8_t *out,
+                            void *d_scratch, hipStream_t st)
+{
+	int *carry = (int *)d_scratch;          // 3 * (w + 2)
+	int *rleft = carry + 3 * (w + 2);        // w + 2
+	const bool alpha_fs = srccomps == 4 && (alphabits == 1 || alphabits == 4);
+	const FsChan cr{0, 0, 3}, cg{1, 0, 2}, cb{2, 0, 3};
+	k_dither_fs<3><<<1, FS_R, 0, st>>>(src, srccomps, w, h, cr, cg, cb, nullptr, carry, alpha_fs ? rleft : nullptr, out);
+	if (int r = hip_check(hipGetLastError(), "dither FS colour kernel")) return r;
+	if (alpha_fs) {
+		const FsChan ca{3, alphabits == 1 ? 1 : 0, 8 - alphabits};
+		k_dither_fs<1><<<1, FS_R, 0, st>>>(src, srccomps, w, h, ca, ca, ca, rleft, carry, nullptr, out);
+		return hip_check(hipGetLastError(), "dither FS alpha kernel");
+	}
+	const long long n = (long long)w * h;
+	k_quantize_none<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(src, srccomps, n, alphabits, 0, 1, out);
+	return hip_check(hipGetLastError(), "quantize alpha kernel");
+}
+
+// ---- host launchers ----
+
+static const int kMaxStates = 31;
+
+size_t rgb565_scratch_bytes(int w, int h)
+{
+	const long long n = (long long)w * h;
+	const long long nchunks = (n + DITHER_L - 1) / DITHER_L;
+	const long long nsuper = (nchunks + DITHER_K - 1) / DITHER_K;
+	const size_t simple = (size_t)(nchunks * (kMaxStates + 1) + nsuper * (kMaxStates + 1) + 256);
+	const size_t fs = (size_t)(4 * (w + 2)) * sizeof(int) + 256;
+	return simple > fs ? simple : fs;
+}
+
+static int run_simple_channel(const uint8_t *src, int srccomps, long long n, DChan ch, uint8_t *out,
+                              uint8_t *scratch, hipStream_t st)
+{
+	const long long nchunks = (n + DITHER_L - 1) / DITHER_L;
+	const long long nsuper = (nchunks + DITHER_K - 1) / DITHER_K;
+	const int S = ch.nstates;
+	uint8_t *maps = scratch;
+	uint8_t *smaps = maps + nchunks * S;
+	uint8_t *sstart = smaps + nsuper * S;
+	uint8_t *cstart = sstart + nsuper;
+	const int T = 256;
+	k_dither_chunk_maps<<<(unsigned)((nchunks * S + T - 1) / T), T, 0, st>>>(src, srccomps, n, ch, maps, nchunks);
+	k_dither_super_maps<<<(unsigned)((nsuper * S + T - 1) / T), T, 0, st>>>(maps, nchunks, ch, smaps, nsuper);
+	const size_t lds = (size_t)(nsuper * S);
+	if (lds <= 60000) {
+		k_dither_super_starts<<<1, T, lds, st>>>(smaps, nsuper, ch, sstart);
+	} else {
+		k_dither_super_starts_global<<<1, 64, 0, st>>>(smaps, nsuper, ch, sstart);
+	}
+	k_dither_chunk_starts<<<(unsigned)((nsuper + T - 1) / T), T, 0, st>>>(maps, nchunks, ch, sstart, nsuper, cstart);
+	k_dither_emit<<<(unsigned)((nchunks + T - 1) / T), T, 0, st>>>(src, srccomps, n, ch, cstart, nchunks, out);
+	return hip_check(hipGetLastError(), "dither SIMPLE kernels");
+}
+
+// rgb565_image with the reference's argument normalisation: srccomps other
+// than 3 means 4 (:1455-1463), alphabits other than 1 and 4 means 8
+// (:1437-1449), dither other than NONE and FLOYDSTEINBERG means SIMPLE
+// (:1419-1431).
+int launch_rgb565(const void *d_src, int srccomps, int w, int h, int alphabits, int dither, void *d_out,
+                  void *d_scratch, hipStream_t st)
+{
+	if (srccomps != 3) srccomps = 4;
+	if (alphabits != 1 && alphabits != 4) alphabits = 8;
+	if (dither != DITHER_NONE && dither != DITHER_FLOYDSTEINBERG) dither = DITHER_SIMPLE;
+	const long long n = (long long)w * h;
+	if (n <= 0) return 0;
+	const uint8_t *src = (const uint8_t *)d_src;
+	uint8_t *out = (uint8_t *)d_out;
+	const unsigned grid = (unsigned)((n + 255) / 256);
+	if (dither == DITHER_NONE) {
+		k_quantize_none<<<grid, 256, 0, st>>>(src, srccomps, n, alphabits, 1, 1, out);
+		return hip_check(hipGetLastError(), "quantize kernel");
+	}
+	if (dither == DITHER_FLOYDSTEINBERG)
+		return launch_rgb565_fs(src, srccomps, w, h, alphabits, out, d_scratch, st);
+	uint8_t *scr = (uint8_t *)d_scratch;
+	const DChan cr{0, 0, 3, 15}, cg{1, 0, 2, 7}, cb{2, 0, 3, 15};
+	if (int r = run_simple_channel(src, srccomps, n, cr, out, scr, st)) return r;
+	if (int r = run_simple_channel(src, srccomps, n, cg, out, scr, st)) return r;
+	if (int r = run_simple_channel(src, srccomps, n, cb, out, scr, st)) return r;
+	if (srccomps == 4 && alphabits == 1) return run_simple_channel(src, srccomps, n, DChan{3, 1, 0, 1}, out, scr, st);
+	if (srccomps == 4 && alphabits == 4) return run_simple_channel(src, srccomps, n, DChan{3, 0, 4, 31}, out, scr, st);
+	k_quantize_none<<<grid, 256, 0, st>>>(src, srccomps, n, alphabits, 0, 1, out);
+	return hip_check(hipGetLastError(), "quantize alpha kernel");
 }
 
 }  // namespace s2tc

@@ -83,7 +83,7 @@ __global__ __launch_bounds__(256) void k_fast(const uint32_t *__restrict__ src, 
 template <int CD, bool RAW>
 __global__ __launch_bounds__(64) void k_normal(const uint32_t *__restrict__ src, int width, int height,
                                                uint8_t *__restrict__ dst, int dxt, int nrandom, int refine,
-                                               unsigned long long seed, int pbw)
+                                               unsigned long long seed, int pbw, unsigned block_base)
 {
 	extern __shared__ __align__(16) uint8_t smem[];
 	const int lane = (int)threadIdx.x;
@@ -100,7 +100,7 @@ __global__ __launch_bounds__(64) void k_normal(const uint32_t *__restrict__ src,
 		tex = src[(size_t)(4 * by + y) * (size_t)width + 4 * bx + x];
 		if (RAW) tex = quant_none_rt(tex, dxt);
 	}
-	const uint32_t bkey = s2tc_prng_block_key((uint64_t)seed, (uint32_t)b);
+	const uint32_t bkey = s2tc_prng_block_key((uint64_t)seed, block_base + (uint32_t)b);
 	const GroupSmem<S2TC_CD> sm = group_smem<S2TC_CD>(smem, g, 16 * pbw);
 	BlockOut o;
 	encode_block_normal<S2TC_CD>(tex, t, w, h, dxt, nrandom, refine, bkey, pbw, sm, o);

@@ -99,9 +99,10 @@ int launch_encode(const EncodeArgs &a, hipStream_t stream)
 	int nrandom = a.nrandom;
 	int pbw = normal_pbw(nrandom);
 	unsigned long long seed = a.seed;
+	unsigned block_base = a.block_index_base;
 	size_t smem = (size_t)kt.normal_smem_per_cand * 16 * pbw;
 	void *args[] = {(void *)&src, (void *)&width, (void *)&height, (void *)&dst, (void *)&dxt,
-	                (void *)&nrandom, (void *)&refine, (void *)&seed, (void *)&pbw};
+	                (void *)&nrandom, (void *)&refine, (void *)&seed, (void *)&pbw, (void *)&block_base};
 	dim3 grid((unsigned)((nb + 3) / 4)), block(64);
 	return hip_check(hipLaunchKernel(kt.normal[raw], grid, block, args, smem, stream), "launch k_normal");
 }
@@ -151,7 +152,7 @@ int s2tc_amd_encode_image(const void *d_src, int src_kind, int width, int height
                           int nrandom, int refine, uint64_t seed, void *stream)
 {
 	if (!d_src || !d_dst) return set_error(S2TC_E_ARG, "null device pointer");
-	EncodeArgs a{d_src, d_dst, src_kind, width, height, dxt, cd, nrandom, refine, seed};
+	EncodeArgs a{d_src, d_dst, src_kind, width, height, dxt, cd, nrandom, refine, seed, 0};
 	return launch_encode(a, (hipStream_t)stream);
 }
 

@@ -27,6 +27,7 @@ struct EncodeArgs {
 	int width, height;
 	int dxt, cd, nrandom, refine;
 	uint64_t seed;
+	uint32_t block_index_base;  // generator block index of the image's first block
 };
 
 int set_error(int code, const char *fmt, ...);
@@ -37,6 +38,11 @@ void normalize_modes(int &dxt, int &cd, int &refine);
 bool is_fast(int cd, int nrandom);
 int normal_pbw(int nrandom);
 int launch_encode(const EncodeArgs &a, hipStream_t stream);
+
+// rgb565_image on the device (dither.hip); d_scratch holds rgb565_scratch_bytes(w, h)
+size_t rgb565_scratch_bytes(int w, int h);
+int launch_rgb565(const void *d_src, int srccomps, int w, int h, int alphabits, int dither, void *d_out,
+                  void *d_scratch, hipStream_t st);
 
 inline int block_bytes(int dxt) { return dxt == DXT1 ? 8 : 16; }
 

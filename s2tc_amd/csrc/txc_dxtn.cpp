@@ -1,0 +1,395 @@
+// txc_dxtn.cpp - the reference-compatible C ABI (txc_dxtn.h, s2tc_algorithm.h).
+//
+// tx_compress_dxtn follows s2tc_libtxc_dxtn.cpp:142-299: environment
+// configuration read on every call, GL format switch, rgb565_image, the
+// raster block loop and the destination row padding rule.  The quantization
+// and the block loop run on the GPU; the host uploads the source, downloads
+// the blocks and lays out the destination rows.  Each call borrows a
+// context (stream + device buffers) from a pool, so concurrent callers on
+// different threads do not share buffers.
+#include "s2tc_internal.h"
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <strings.h>
+#include <vector>
+
+#include "s2tc_algorithm.h"
+#include "txc_dxtn.h"
+
+namespace s2tc {
+
+struct DevBuf {
+	void *p = nullptr;
+	size_t cap = 0;
+	int ensure(size_t n)
+	{
+		if (n <= cap) return 0;
+		if (p) (void)hipFree(p);
+		p = nullptr;
+		cap = 0;
+		n = (n + 4095) & ~(size_t)4095;
+		if (int r = hip_check(hipMalloc(&p, n), "hipMalloc")) return r;
+		cap = n;
+		return 0;
+	}
+};
+
+struct Ctx {
+	hipStream_t stream = nullptr;
+	DevBuf src, q, out, scratch;
+	std::vector<uint8_t> host_out;
+};
+
+static std::mutex g_pool_mu;
+static std::vector<Ctx *> g_pool;
+
+static Ctx *ctx_acquire()
+{
+	{
+		std::lock_guard<std::mutex> lk(g_pool_mu);
+		if (!g_pool.empty()) {
+			Ctx *c = g_pool.back();
+			g_pool.pop_back();
+			return c;
+		}
+	}
+	Ctx *c = new Ctx;
+	if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+		delete c;
+		return nullptr;
+	}
+	return c;
+}
+
+static void ctx_release(Ctx *c)
+{
+	std::lock_guard<std::mutex> lk(g_pool_mu);
+	g_pool.push_back(c);
+}
+
+struct CtxGuard {
+	Ctx *c;
+	CtxGuard() : c(ctx_acquire()) {}
+	~CtxGuard()
+	{
+		if (c) ctx_release(c);
+	}
+};
+
+// Environment configuration of s2tc_libtxc_dxtn.cpp:156-216.
+struct EnvConfig {
+	int cd = CD_WAVG, nrandom = -1, refine = REFINE_ALWAYS, dither = DITHER_SIMPLE;
+	uint64_t seed = 0;
+};
+
+static EnvConfig read_env()
+{
+	EnvConfig e;
+	if (const char *v = getenv("S2TC_DITHER_MODE")) {
+		if (!strcasecmp(v, "NONE")) e.dither = DITHER_NONE;
+		else if (!strcasecmp(v, "SIMPLE")) e.dither = DITHER_SIMPLE;
+		else if (!strcasecmp(v, "FLOYDSTEINBERG")) e.dither = DITHER_FLOYDSTEINBERG;
+		else fprintf(stderr, "Invalid dither mode: %s\n", v);
+	}
+	if (const char *v = getenv("S2TC_COLORDIST_MODE")) {
+		static const char *names[] = {"RGB", "YUV", "SRGB", "SRGB_MIXED", "AVG", "WAVG", "W0AVG", "NORMALMAP"};
+		int found = -1;
+		for (int i = 0; i < 8; ++i)
+			if (!strcasecmp(v, names[i])) found = i;
+		if (found >= 0) e.cd = found;
+		else fprintf(stderr, "Invalid color dist mode: %s\n", v);
+	}
+	if (const char *v = getenv("S2TC_RANDOM_COLORS")) e.nrandom = atoi(v);
+	if (const char *v = getenv("S2TC_REFINE_COLORS")) {
+		if (!strcasecmp(v, "NEVER")) e.refine = REFINE_NEVER;
+		else if (!strcasecmp(v, "ALWAYS")) e.refine = REFINE_ALWAYS;
+		else if (!strcasecmp(v, "LOOP")) e.refine = REFINE_LOOP;
+		else fprintf(stderr, "Invalid refinement mode: %s\n", v);
+	}
+	if (const char *v = getenv("S2TC_RANDOM_SEED")) e.seed = strtoull(v, nullptr, 0);
+	return e;
+}
+
+// Whole-image encode of host data on the context's stream; writes the
+// blocks in raster order to c->host_out.
+static int encode_host_image(Ctx *c, int srccomps, int width, int height, const uint8_t *src, int dxt, int cd,
+                             int nrandom, int refine, int dither, uint64_t seed)
+{
+	if (srccomps != 3) srccomps = 4;
+	const size_t n = (size_t)width * height;
+	const size_t nb = (size_t)((width + 3) / 4) * ((height + 3) / 4);
+	const size_t obytes = nb * block_bytes(dxt);
+	const int alphabits = dxt == DXT1 ? 1 : dxt == DXT3 ? 4 : 8;
+	if (int r = c->src.ensure(n * srccomps)) return r;
+	if (int r = c->out.ensure(obytes)) return r;
+	if (int r = hip_check(hipMemcpyAsync(c->src.p, src, n * srccomps, hipMemcpyHostToDevice, c->stream), "H2D"))
+		return r;
+	EncodeArgs a{};
+	a.dst = c->out.p;
+	a.width = width;
+	a.height = height;
+	a.dxt = dxt;
+	a.cd = cd;
+	a.nrandom = nrandom;
+	a.refine = refine;
+	a.seed = seed;
+	if (dither == DITHER_NONE && srccomps == 4) {
+		a.src = c->src.p;  // quantization fused into the encoder's loads
+		a.src_kind = S2TC_SRC_RGBA8;
+	} else {
+		if (int r = c->q.ensure(n * 4)) return r;
+		if (int r = c->scratch.ensure(rgb565_scratch_bytes(width, height))) return r;
+		if (int r = launch_rgb565(c->src.p, srccomps, width, height, alphabits, dither, c->q.p, c->scratch.p,
+		                          c->stream))
+			return r;
+		a.src = c->q.p;
+		a.src_kind = S2TC_SRC_QUANTIZED;
+	}
+	if (int r = launch_encode(a, c->stream)) return r;
+	c->host_out.resize(obytes);
+	if (int r = hip_check(hipMemcpyAsync(c->host_out.data(), c->out.p, obytes, hipMemcpyDeviceToHost, c->stream),
+	                      "D2H"))
+		return r;
+	return hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+}
+
+// ---- per-block entry points of s2tc_encode_block_func ----
+
+static thread_local uint32_t t_block_index = 0;
+
+static void encode_one_block(unsigned char *out, const unsigned char *rgba, int iw, int w, int h, int nrandom,
+                             int dxt, int cd, int fast, int refine)
+{
+	CtxGuard g;
+	Ctx *c = g.c;
+	const uint32_t bidx = t_block_index++;
+	if (!c) {
+		fprintf(stderr, "s2tc: cannot create a HIP stream\n");
+		return;
+	}
+	if (w < 1 || h < 1 || w > 4 || h > 4) return;
+	uint8_t tile[64];
+	for (int y = 0; y < h; ++y) memcpy(tile + y * w * 4, rgba + (size_t)y * iw * 4, (size_t)w * 4);
+	const int bs = block_bytes(dxt);
+	int r = c->src.ensure(64);
+	if (!r) r = c->out.ensure(16);
+	if (!r) r = hip_check(hipMemcpyAsync(c->src.p, tile, (size_t)w * h * 4, hipMemcpyHostToDevice, c->stream), "H2D");
+	if (!r) {
+		// A one-block image whose only block has index bidx in the generator.
+		// The selection-time mode is kept: quick mode runs with nrandom < 0,
+		// MODE_NORMAL with max(nrandom, 0) extra candidates.
+		EncodeArgs a{c->src.p, c->out.p, S2TC_SRC_QUANTIZED, w, h, dxt, cd,
+		             fast ? -1 : (nrandom > 0 ? nrandom : 0), refine, 0, 0};
+		const char *sv = getenv("S2TC_RANDOM_SEED");
+		a.seed = sv ? strtoull(sv, nullptr, 0) : 0;
+		a.block_index_base = bidx;
+		r = launch_encode(a, c->stream);
+	}
+	uint8_t blk[16];
+	if (!r) r = hip_check(hipMemcpyAsync(blk, c->out.p, bs, hipMemcpyDeviceToHost, c->stream), "D2H");
+	if (!r) r = hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+	if (r) {
+		fprintf(stderr, "s2tc: %s\n", s2tc_amd_last_error());
+		return;
+	}
+	memcpy(out, blk, bs);
+}
+
+template <int DXT, int CD, int FAST, int REFINE>
+static void block_fn(unsigned char *out, const unsigned char *rgba, int iw, int w, int h, int nrandom)
+{
+	encode_one_block(out, rgba, iw, w, h, nrandom, DXT, CD, FAST, REFINE);
+}
+
+template <int DXT, int CD, int FAST>
+static s2tc_encode_block_func_t pick_refine(int refine)
+{
+	switch (refine) {
+	case REFINE_NEVER: return block_fn<DXT, CD, FAST, REFINE_NEVER>;
+	case REFINE_LOOP: return block_fn<DXT, CD, FAST, REFINE_LOOP>;
+	default: return block_fn<DXT, CD, FAST, REFINE_ALWAYS>;
+	}
+}
+
+template <int DXT, int CD>
+static s2tc_encode_block_func_t pick_mode(int nrandom, int refine)
+{
+	if (CD == CD_NORMALMAP || nrandom >= 0) return pick_refine<DXT, CD, 0>(refine);
+	return pick_refine<DXT, CD, CD == CD_NORMALMAP ? 0 : 1>(refine);
+}
+
+template <int CD>
+static s2tc_encode_block_func_t pick_dxt(int dxt, int nrandom, int refine)
+{
+	switch (dxt) {
+	case DXT1: return pick_mode<DXT1, CD>(nrandom, refine);
+	case DXT3: return pick_mode<DXT3, CD>(nrandom, refine);
+	default: return pick_mode<DXT5, CD>(nrandom, refine);
+	}
+}
+
+}  // namespace s2tc
+
+using namespace s2tc;
+
+extern "C" {
+
+void tx_compress_dxtn(GLint srccomps, GLint width, GLint height, const GLubyte *srcPixData, GLenum destformat,
+                      GLubyte *dest, GLint dstRowStride)
+{
+	const EnvConfig e = read_env();
+	int dxt, k;
+	switch (destformat) {
+	case 0x83F0: /* GL_COMPRESSED_RGB_S3TC_DXT1_EXT */
+	case 0x83F1: /* GL_COMPRESSED_RGBA_S3TC_DXT1_EXT */
+		dxt = s2tc::DXT1;
+		k = 2;
+		break;
+	case 0x83F2: dxt = s2tc::DXT3; k = 4; break;
+	case 0x83F3: dxt = s2tc::DXT5; k = 4; break;
+	default:
+		fprintf(stderr, "libdxtn: Bad dstFormat %d in tx_compress_dxtn\n", destformat);
+		return;
+	}
+	if (width <= 0 || height <= 0) return;
+	CtxGuard g;
+	if (!g.c) {
+		fprintf(stderr, "s2tc: cannot create a HIP stream\n");
+		return;
+	}
+	if (int r = encode_host_image(g.c, srccomps, width, height, srcPixData, dxt, e.cd, e.nrandom, e.refine, e.dither,
+	                              e.seed)) {
+		fprintf(stderr, "s2tc: tx_compress_dxtn failed (%d): %s\n", r, s2tc_amd_last_error());
+		return;
+	}
+	// destination layout with dstRowDiff (s2tc_libtxc_dxtn.cpp:243, :261, :279)
+	const int bs = block_bytes(dxt);
+	const int bw = (width + 3) / 4, bh = (height + 3) / 4;
+	const long rowdiff = dstRowStride >= width * k ? (long)dstRowStride - (long)((width + 3) & ~3) * k : 0;
+	GLubyte *blk = dest;
+	for (int by = 0; by < bh; ++by) {
+		memcpy(blk, g.c->host_out.data() + (size_t)by * bw * bs, (size_t)bw * bs);
+		blk += (long)bw * bs + rowdiff;
+	}
+}
+
+void rgb565_image(unsigned char *out, const unsigned char *rgba, int w, int h, int srccomps, int alphabits,
+                  enum DitherMode dither)
+{
+	if (w <= 0 || h <= 0) return;
+	CtxGuard g;
+	Ctx *c = g.c;
+	if (!c) {
+		fprintf(stderr, "s2tc: cannot create a HIP stream\n");
+		return;
+	}
+	const int sc = srccomps == 3 ? 3 : 4;
+	const size_t n = (size_t)w * h;
+	int r = c->src.ensure(n * sc);
+	if (!r) r = c->q.ensure(n * 4);
+	if (!r) r = c->scratch.ensure(rgb565_scratch_bytes(w, h));
+	if (!r) r = hip_check(hipMemcpyAsync(c->src.p, rgba, n * sc, hipMemcpyHostToDevice, c->stream), "H2D");
+	if (!r) r = launch_rgb565(c->src.p, sc, w, h, alphabits, (int)dither, c->q.p, c->scratch.p, c->stream);
+	if (!r) r = hip_check(hipMemcpyAsync(out, c->q.p, n * 4, hipMemcpyDeviceToHost, c->stream), "D2H");
+	if (!r) r = hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+	if (r) fprintf(stderr, "s2tc: rgb565_image failed (%d): %s\n", r, s2tc_amd_last_error());
+}
+
+s2tc_encode_block_func_t s2tc_encode_block_func(enum DxtMode dxt, ColorDistMode cd, int nrandom,
+                                                enum RefinementMode refine)
+{
+	switch ((int)cd) {
+	case CD_RGB: return pick_dxt<CD_RGB>(dxt, nrandom, refine);
+	case CD_YUV: return pick_dxt<CD_YUV>(dxt, nrandom, refine);
+	case CD_SRGB: return pick_dxt<CD_SRGB>(dxt, nrandom, refine);
+	case CD_SRGB_MIXED: return pick_dxt<CD_SRGB_MIXED>(dxt, nrandom, refine);
+	case CD_AVG: return pick_dxt<CD_AVG>(dxt, nrandom, refine);
+	case CD_W0AVG: return pick_dxt<CD_W0AVG>(dxt, nrandom, refine);
+	case CD_NORMALMAP: return pick_dxt<CD_NORMALMAP>(dxt, nrandom, refine);
+	default: return pick_dxt<CD_WAVG>(dxt, nrandom, refine);
+	}
+}
+
+void s2tc_amd_set_block_index(unsigned int index) { t_block_index = index; }
+
+// ---- S2TC decoders (s2tc_libtxc_dxtn.cpp:35-140) ----
+
+static inline void expand565(unsigned c, GLubyte *t)
+{
+	unsigned r = (c >> 11) & 0x1f, g = (c >> 5) & 0x3f, b = c & 0x1f;
+	t[0] = (GLubyte)((r << 3) | (r >> 2));
+	t[1] = (GLubyte)((g << 2) | (g >> 4));
+	t[2] = (GLubyte)((b << 3) | (b >> 2));
+}
+
+static inline const GLubyte *block_at(GLint stride, const GLubyte *pix, GLint i, GLint j, int bs)
+{
+	return pix + ((size_t)((stride + 3) >> 2) * (size_t)(j >> 2) + (size_t)(i >> 2)) * bs;
+}
+
+// colour of an S2TC block: 0 -> c0, 1 -> c1, 2/3 -> checkerboard of c0/c1;
+// with dxt1_black, index 3 with c1 >= c0 is black (transparent for RGBA).
+static inline unsigned s2tc_color(const GLubyte *cb, GLint i, GLint j, bool dxt1, bool &black)
+{
+	const unsigned c0 = cb[0] | (cb[1] << 8), c1 = cb[2] | (cb[3] << 8);
+	const int idx = (cb[4 + (j & 3)] >> (2 * (i & 3))) & 3;
+	black = false;
+	if (idx == 0) return c0;
+	if (idx == 1) return c1;
+	if (dxt1 && idx == 3 && c1 >= c0) {
+		black = true;
+		return 0;
+	}
+	return ((i ^ j) & 1) ? c1 : c0;
+}
+
+void fetch_2d_texel_rgb_dxt1(GLint srcRowStride, const GLubyte *pixdata, GLint i, GLint j, GLvoid *texel)
+{
+	GLubyte *t = (GLubyte *)texel;
+	bool black;
+	expand565(s2tc_color(block_at(srcRowStride, pixdata, i, j, 8), i, j, true, black), t);
+	t[3] = 255;
+}
+
+void fetch_2d_texel_rgba_dxt1(GLint srcRowStride, const GLubyte *pixdata, GLint i, GLint j, GLvoid *texel)
+{
+	GLubyte *t = (GLubyte *)texel;
+	bool black;
+	expand565(s2tc_color(block_at(srcRowStride, pixdata, i, j, 8), i, j, true, black), t);
+	t[3] = black ? 0 : 255;
+}
+
+void fetch_2d_texel_rgba_dxt3(GLint srcRowStride, const GLubyte *pixdata, GLint i, GLint j, GLvoid *texel)
+{
+	GLubyte *t = (GLubyte *)texel;
+	const GLubyte *blk = block_at(srcRowStride, pixdata, i, j, 16);
+	bool black;
+	expand565(s2tc_color(blk + 8, i, j, false, black), t);
+	const int a = (blk[(j & 3) * 2 + ((i & 3) >> 1)] >> (4 * (i & 1))) & 0x0f;
+	t[3] = (GLubyte)(a | (a << 4));
+}
+
+void fetch_2d_texel_rgba_dxt5(GLint srcRowStride, const GLubyte *pixdata, GLint i, GLint j, GLvoid *texel)
+{
+	GLubyte *t = (GLubyte *)texel;
+	const GLubyte *blk = block_at(srcRowStride, pixdata, i, j, 16);
+	bool black;
+	expand565(s2tc_color(blk + 8, i, j, false, black), t);
+	unsigned a = blk[0];
+	const unsigned a1 = blk[1];
+	uint64_t bits = 0;
+	for (int q = 0; q < 6; ++q) bits |= (uint64_t)blk[2 + q] << (8 * q);
+	const int ab = (int)((bits >> (3 * ((j & 3) * 4 + (i & 3)))) & 7);
+	// index 6 is 0 and 7 is 255 in 6-value mode (a1 >= a0); otherwise 6 falls
+	// through to 7's test and both fall through to the checkerboard (:130-137)
+	if (ab == 1) a = a1;
+	else if (ab == 6 && a1 >= a) a = 0;
+	else if ((ab == 6 || ab == 7) && a1 >= a) a = 255;
+	else if (ab != 0 && ((i ^ j) & 1)) a = a1;
+	t[3] = (GLubyte)a;
+}
+
+}  // extern "C"

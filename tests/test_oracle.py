@@ -1,0 +1,232 @@
+"""Pins of the CPU oracle (oracle/s2tc_oracle.c).
+
+1. Golden fixtures: every case in tests/golden/*.npz was produced by the
+   compiled reference (oracle/_ref); the oracle must reproduce all of them.
+2. Live comparison with the compiled reference where it is built.
+3. Closed forms and invariants that hold independently of both: distance
+   formulas on hand-computed pairs, srgb_get_y exhaustively, the
+   all-transparent DXT1 block, endpoint order and index alphabet rules.
+A planted mistake in the oracle (e.g. a swapped SRGB argument order or a
+dropped rounding term) fails several of these.
+"""
+import itertools
+import math
+import os
+
+import numpy as np
+import pytest
+
+import imgutil
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+NAMES = ["RGB", "YUV", "SRGB", "SRGB_MIXED", "AVG", "WAVG", "W0AVG", "NORMALMAP"]
+
+
+def load(name):
+    return np.load(os.path.join(GOLDEN, name), allow_pickle=False)
+
+
+def test_oracle_matches_golden_blocks(oracle):
+    z = load("blocks.npz")
+    n = 0
+    for key in z.files:
+        if not key.startswith("out_"):
+            continue
+        _, dxt, i, cd, nr, refine = key.split("_")
+        dxt, i, cd, nr, refine = int(dxt), int(i), int(cd), int(nr), int(refine)
+        w, h = z[f"wh_{dxt}_{i}"]
+        got = oracle.encode_quantized(z[f"in_{dxt}_{i}"], int(w), int(h), dxt, cd, nr, refine, seed=1000 + i)
+        assert np.array_equal(got, z[key]), key
+        n += 1
+    assert n == 3 * 8 * 8 * 4 * 3
+
+
+def test_oracle_matches_golden_dither(oracle):
+    z = load("dither.npz")
+    for key in z.files:
+        if not key.startswith("q_"):
+            continue
+        _, w, h, sc, ab, d = key.split("_")
+        w, h, sc, ab, d = map(int, (w, h, sc, ab, d))
+        got = oracle.rgb565_image(z[f"src_{w}_{h}_{sc}"], w, h, sc, ab, d)
+        assert np.array_equal(got, z[key]), key
+
+
+def test_oracle_matches_golden_compress(oracle):
+    z = load("compress.npz")
+    for key in z["cases"]:
+        w, h, fmt, stride, d, cd, nr, refine = map(int, str(key).split("_"))
+        r, got = oracle.compress(4, w, h, z[f"src_{w}_{h}"], fmt, stride, cd, nr, refine, d, seed=77, fill=0xAB)
+        assert r == 0
+        assert np.array_equal(got, z["dst_" + str(key)]), key
+
+
+def test_oracle_matches_compiled_reference_live(oracle, reference):
+    rng = np.random.default_rng(99)
+    for (w, h), dxt, kind in itertools.product([(11, 6), (4, 4), (1, 2)], range(3),
+                                               ["random", "extreme", "speckle"]):
+        q = imgutil.random_quantized(rng, w, h, dxt, kind)
+        for cd, nr, refine in itertools.product(range(8), [-1, 0, 7], range(3)):
+            a = oracle.encode_quantized(q, w, h, dxt, cd, nr, refine, seed=5)
+            b = reference.encode_quantized(q, w, h, dxt, cd, nr, refine, seed=5)
+            assert np.array_equal(a, b), (w, h, dxt, kind, cd, nr, refine)
+    assert reference.lib.ref_rand_calls() > 0, "rand() of the reference did not reach the harness"
+
+
+# ---- closed forms ----
+
+def test_distance_closed_forms(oracle):
+    d = oracle.color_dist
+    # AVG 4dr^2+dg^2+4db^2, W0AVG, WAVG 4dr^2+4dg^2+db^2
+    assert d(4, (3, 10, 5), (1, 4, 2)) == 4 * 4 + 36 + 4 * 9
+    assert d(6, (3, 10, 5), (1, 4, 2)) == 4 + 36 + 9
+    assert d(5, (3, 10, 5), (1, 4, 2)) == 4 * 4 + 4 * 36 + 9
+    # YUV by hand: dr=2, dg=6, db=3: y=2*60+6*59+3*22=540, u=404-540=-136, v=606-540=66
+    y, u, v = 540, -136, 66
+    assert d(1, (3, 10, 5), (1, 4, 2)) == 2 * y * y + ((u * u + 4) >> 3) + ((v * v + 8) >> 4)
+    # RGB: y=2*42+6*72+3*14=558, u=404-558=-154, v=606-558=48
+    y, u, v = 558, -154, 48
+    assert d(0, (3, 10, 5), (1, 4, 2)) == 2 * y * y + ((u * u + 4) >> 3) + ((v * v + 8) >> 4)
+    # every distance of a colour with itself is 0
+    for cd in range(8):
+        assert d(cd, (7, 33, 19), (7, 33, 19)) == 0
+
+
+def _shrr(a, n):
+    return (a + (1 << (n - 1))) >> n
+
+
+def _wrap32(x):
+    x &= 0xFFFFFFFF
+    return x - (1 << 32) if x >= 1 << 31 else x
+
+
+def test_srgb_wraps_and_is_asymmetric(oracle):
+    """SRGB overflows int32 for saturated pairs; the value is the wrapped one."""
+    a, b = (0, 60, 0), (30, 0, 30)
+    dr, dg, db = 0 - 900, 3600 - 0, 0 - 900
+    y = dr * 84 + dg * 72 + db * 28
+    u = dr * 409 - y
+    v = db * 409 - y
+    sy = _wrap32(_shrr(y, 3) * _shrr(y, 4))
+    su = _wrap32(_shrr(u, 3) * _shrr(u, 4))
+    sv = _wrap32(_shrr(v, 3) * _shrr(v, 4))
+    expect = _wrap32(_shrr(sy, 4) + _shrr(su, 8) + _shrr(sv, 9))
+    assert oracle.color_dist(2, a, b) == expect
+    assert expect < 0
+    assert oracle.color_dist(2, a, b) != oracle.color_dist(2, b, a)
+
+
+def test_srgb_get_y_exhaustive(oracle):
+    """(int)(sqrtf(37*(84r^2+72g^2+28b^2)) + 0.5f) for all 65536 colours,
+    against float32 numpy (IEEE sqrt, round-to-nearest add)."""
+    r, g, b = np.meshgrid(np.arange(32), np.arange(64), np.arange(32), indexing="ij")
+    y = (37 * (84 * r * r + 72 * g * g + 28 * b * b)).astype(np.float32)
+    expect = (np.sqrt(y, dtype=np.float32) + np.float32(0.5)).astype(np.int32)
+    for (ri, gi, bi) in [(0, 0, 0), (31, 63, 31), (1, 2, 3), (17, 0, 30)]:
+        assert oracle.lib.oracle_srgb_get_y(ri, gi, bi) == expect[ri, gi, bi]
+    # full sweep through SRGB_MIXED distances to black: Y^2*8 + SHRR(U^2,1) + SHRR(V^2,2)
+    sample = np.random.default_rng(0).integers(0, 65536, 3000)
+    for s in sample:
+        ri, gi, bi = s >> 11, (s >> 5) & 63, s & 31
+        Y = int(expect[ri, gi, bi])
+        U, V = ri * 191 - Y, bi * 191 - Y
+        assert oracle.color_dist(3, (ri, gi, bi), (0, 0, 0)) == 8 * Y * Y + _shrr(U * U, 1) + _shrr(V * V, 2)
+
+
+def test_normalmap_against_float32_numpy(oracle):
+    rng = np.random.default_rng(1)
+    f32 = np.float32
+    for _ in range(2000):
+        a = rng.integers(0, [32, 64, 32])
+        b = rng.integers(0, [32, 64, 32])
+
+        def nv(c):
+            v = [f32(c[0]) / f32(31.0) * f32(2) - f32(1), f32(c[1]) / f32(63.0) * f32(2) - f32(1),
+                 f32(c[2]) / f32(31.0) * f32(2) - f32(1)]
+            n = v[0] * v[0] + v[1] * v[1] + v[2] * v[2]
+            if n > 0:
+                n = f32(1.0) / np.sqrt(f32(n))
+                v = [v[0] * n, v[1] * n, v[2] * n]
+            return v
+        ca, cb = nv(a), nv(b)
+        s = (cb[0] - ca[0]) * (cb[0] - ca[0]) + (cb[1] - ca[1]) * (cb[1] - ca[1]) + (cb[2] - ca[2]) * (cb[2] - ca[2])
+        assert oracle.color_dist(7, tuple(a), tuple(b)) == int(f32(100000) * f32(s))
+
+
+def test_all_transparent_dxt1_quick_block(oracle):
+    """dummy endpoints (31,63,31)/(0,0,0), all indices 3, swap to c0 <= c1:
+    00 00 ff ff | ff ff ff ff."""
+    q = np.zeros(64, np.uint8)   # alpha bit 0 everywhere
+    for cd in range(7):
+        out = oracle.encode_quantized(q, 4, 4, 0, cd, -1, 1)
+        assert out.tolist() == [0, 0, 0xff, 0xff, 0xff, 0xff, 0xff, 0xff], cd
+
+
+def _unpack(blk, dxt):
+    cb = blk[8:] if dxt else blk
+    c0 = int(cb[0]) | int(cb[1]) << 8
+    c1 = int(cb[2]) | int(cb[3]) << 8
+    idx = int.from_bytes(bytes(cb[4:8]), "little")
+    return c0, c1, [(idx >> (2 * i)) & 3 for i in range(16)]
+
+
+@pytest.mark.parametrize("dxt", range(3))
+def test_s2tc_invariants(oracle, dxt):
+    """DXT1: c0 <= c1 (3-colour mode); DXT3/5: c0 > c1; no index 2; DXT1 index 3
+    only where alpha is 0; DXT5: a0 < a1 and alpha indices in {0,1,6,7}."""
+    rng = np.random.default_rng(dxt)
+    for kind in ("random", "smooth", "speckle", "extreme"):
+        q = imgutil.random_quantized(rng, 8, 8, dxt, kind)
+        qa = q.reshape(8, 8, 4)
+        for cd, nr, refine in itertools.product(range(8), [-1, 0, 4], range(3)):
+            out = oracle.encode_quantized(q, 8, 8, dxt, cd, nr, refine, seed=3)
+            bs = 8 if dxt == 0 else 16
+            for b in range(4):
+                blk = out[b * bs:(b + 1) * bs]
+                c0, c1, idx = _unpack(blk, dxt)
+                assert 2 not in idx
+                if dxt == 0:
+                    assert c0 <= c1
+                    by, bx = divmod(b, 2)
+                    for i, v in enumerate(idx):
+                        if v == 3:
+                            assert qa[4 * by + i // 4, 4 * bx + i % 4, 3] == 0
+                else:
+                    assert c0 > c1
+                if dxt == 2:
+                    assert blk[0] < blk[1]
+                    bits = int.from_bytes(bytes(blk[2:8]), "little")
+                    assert {(bits >> (3 * i)) & 7 for i in range(16)} <= {0, 1, 6, 7}
+
+
+def test_reduce_is_brute_force_argmin(oracle):
+    """With all-16 candidates and REFINE_NEVER the chosen endpoint pair
+    minimises sum_k min(d(c_i, p_k), d(c_j, p_k)) over all pairs of block
+    colours (first pair in (i, j) order on ties): checked by brute force for
+    WAVG on random blocks."""
+    rng = np.random.default_rng(4)
+    for _ in range(50):
+        q = imgutil.random_quantized(rng, 4, 4, 1, "smooth")
+        qa = q.reshape(4, 4, 4)
+        cols = [tuple(int(v) for v in qa[y, x, :3]) for x in range(4) for y in range(4)]   # x-major
+        def d(a, b):
+            return 4 * (a[0] - b[0]) ** 2 + 4 * (a[1] - b[1]) ** 2 + (a[2] - b[2]) ** 2
+        best, bi, bj = None, 0, 1
+        for i in range(16):
+            for j in range(i + 1, 16):
+                s = sum(min(d(cols[i], p), d(cols[j], p)) for p in cols)
+                if best is None or s < best:
+                    best, bi, bj = s, i, j
+        e0, e1 = cols[bi], cols[bj]
+        if e0 == e1:
+            v = (e1[0] << 11 | e1[1] << 5 | e1[2])
+            v = v - 1 if (e0[0] << 11 | e0[1] << 5 | e0[2]) == 0xffff else v + 1
+            e1 = (v >> 11, (v >> 5) & 63, v & 31)
+        v0 = e0[0] << 11 | e0[1] << 5 | e0[2]
+        v1 = e1[0] << 11 | e1[1] << 5 | e1[2]
+        if v0 < v1:
+            v0, v1 = v1, v0
+        out = oracle.encode_quantized(q, 4, 4, 1, 5, 0, 0)
+        c0, c1, _ = _unpack(out, 1)
+        assert (c0, c1) == (v0, v1)
