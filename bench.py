@@ -1,0 +1,217 @@
+#!/usr/bin/env python3
+"""Benchmark: S2TC encoding throughput on one MI355X.
+
+Default workload ("batch"): 2000 RGBA8 textures of mixed power-of-two sizes
+(4..2048 per side) with their full mip chains down to 1x1 - 16467 images,
+7.54e8 texels - mixed DXT1/DXT3/DXT5 and all eight distance modes, encoded
+bit-exactly (DITHER_NONE quantization) as one persistent hipGraph launch.
+Sources and outputs are resident in HBM before the timed region.  A step is
+one launch of the whole batch.
+
+Printed (one JSON line): Mtexels/s of the batched launch (value), the same
+jobs as plain per-image launches, the roofline of the dominant kernel class,
+and the compiled reference's CPU throughput on a sample of the same jobs.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload batch|tiles4096|normal8192|sinus256]
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
+VALU_PEAK_TOPS = 39.3          # int32 lane-ops/s: 256 CUs x 64 lanes x 2.4 GHz
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def batch_items(seed, count, serve_mix=False):
+    from s2tc_amd import workloads
+    jobs = workloads.batch_jobs(seed, count, serve_mix)
+    items = [dict(rgba=lv["rgba"], width=lv["width"], height=lv["height"], dxt=t["dxt"], cd=t["cd"],
+                  nrandom=t["nrandom"], refine=t["refine"], seed=lv["seed"]) for _, _, t, lv in jobs.jobs()]
+    return items
+
+
+def image_items(name):
+    from s2tc_amd import workloads
+    wl = workloads.IMAGE_WORKLOADS[name]
+    src = workloads.make_image_source(wl)
+    if wl["srccomps"] == 3:
+        import numpy as np
+        rgba = np.full(wl["width"] * wl["height"] * 4, 255, np.uint8)
+        rgba.reshape(-1, 4)[:, :3] = src.reshape(-1, 3)
+        src = rgba
+    return [dict(rgba=src, width=wl["width"], height=wl["height"], dxt=wl["dxt"], cd=wl["cd"],
+                 nrandom=wl["nrandom"], refine=wl["refine"], seed=wl["seed"])]
+
+
+def cpu_baseline(items, budget_s=12.0):
+    """The compiled reference (oracle/_ref), threads over all host cores, on a
+    sample of the workload's jobs: jobs are taken in a fixed stride order
+    until about budget_s of single-thread-equivalent work is reached."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    from oracle import Reference, reference_available
+    if not reference_available():
+        return None
+    ref = Reference()
+
+    class RefJob(ctypes.Structure):
+        _fields_ = [("q", ctypes.c_void_p), ("dest", ctypes.c_void_p), ("width", ctypes.c_int),
+                    ("height", ctypes.c_int), ("dxt", ctypes.c_int), ("cd", ctypes.c_int),
+                    ("nrandom", ctypes.c_int), ("refine", ctypes.c_int), ("seed", ctypes.c_uint64)]
+    ref.lib.ref_encode_jobs_mt.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+    ref.lib.ref_encode_jobs_mt.restype = ctypes.c_int
+    ncores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    # stride sample over the jobs, grown until the sample is big enough
+    order = list(range(0, len(items), 7)) + [i for i in range(len(items)) if i % 7]
+    keep, arr, texels, t_used = [], [], 0, 0.0
+    chunk = max(1, len(items) // 50)
+    pos = 0
+    while pos < len(order) and t_used < budget_s / 2:
+        sel = order[pos:pos + chunk]
+        pos += chunk
+        batch, bkeep = [], []
+        for i in sel:
+            it = items[i]
+            w, h = it["width"], it["height"]
+            q = ref.rgb565_image(it["rgba"], w, h, 4, {0: 1, 1: 4, 2: 8}[it["dxt"]], 0)
+            dst = np.zeros(((w + 3) // 4) * ((h + 3) // 4) * (8 if it["dxt"] == 0 else 16), np.uint8)
+            bkeep += [q, dst]
+            batch.append(RefJob(q.ctypes.data, dst.ctypes.data, w, h, it["dxt"], it["cd"], it["nrandom"],
+                                it["refine"], it["seed"]))
+            texels += w * h
+        ja = (RefJob * len(batch))(*batch)
+        t0 = time.perf_counter()
+        n = ref.lib.ref_encode_jobs_mt(ctypes.cast(ja, ctypes.c_void_p), len(batch), ncores)
+        t_used += time.perf_counter() - t0
+        keep += bkeep
+    return dict(value=texels / t_used / 1e6, unit="Mtexels/s", cores=int(n), kind="reference",
+                sample=f"{texels} texels ({pos} of {len(items)} images, every 7th first) of the same workload, "
+                       f"DITHER_NONE quantization + the reference's block loop, {n} threads, {t_used:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="batch")
+    ap.add_argument("--schedule", default="", help="wg_per_cu,streams (skips the autotuner)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0,
+                    help="only run a plain device copy of this many MiB (PMC calibration) and exit")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (asked for {a.gpus}); multi-GPU sharding is "
+              "described in DESIGN.md, not run.", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import s2tc_amd as s
+    L = s.lib()
+    assert L.s2tc_amd_device_count() > 0, "no GPU visible"
+    if a.calibrate_copy:
+        n = a.calibrate_copy << 20
+        src, dst = s.DeviceBuffer(n), s.DeviceBuffer(n)
+        src.fill(1)
+        L.s2tc_amd_memcpy_d2d_kernel.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]
+        for _ in range(3):
+            s.check(L.s2tc_amd_memcpy_d2d_kernel(dst.ptr, src.ptr, n, None))
+        s.synchronize()
+        print(json.dumps({"calibrate_copy_bytes": n}))
+        return
+
+    t0 = time.time()
+    items = batch_items(3, 2000) if a.workload == "batch" else image_items(a.workload)
+    log(f"workload {a.workload}: {len(items)} images, generated in {time.time() - t0:.1f} s")
+    imgs = s.DeviceImages(items)
+    texels = imgs.texels
+    batch = s.Batch(imgs)
+    if a.schedule:
+        wg, ns = (int(v) for v in a.schedule.split(","))
+        batch.set_schedule(wg, ns)
+        tuned_ms = None
+    else:
+        tuned_ms = batch.autotune(reps=2)
+    sched = batch.schedule()
+    log(f"schedule wg_per_cu={sched[0]} streams={sched[1]} (autotuned {tuned_ms} ms)")
+
+    for _ in range(a.warmup):
+        batch.launch()
+    s.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        batch.launch()
+    s.synchronize()
+    dt = (time.perf_counter() - t0) / a.steps
+    value = texels / dt / 1e6
+
+    # per-class kernel times (HIP events on the launch stream), repeated
+    classes = None
+    for _ in range(max(1, min(a.steps, 3))):
+        ct = batch.class_times()
+        if classes is None:
+            classes = ct
+        else:
+            for c, n in zip(classes, ct):
+                c["ms"] = min(c["ms"], n["ms"])
+    dom = max(classes, key=lambda c: c["ms"])
+    names = s.COLORDIST_NAMES
+    kname = (f"k_normal_batch<{dom['cd']}>" if dom["normal"] else f"k_fast_batch<{dom['cd']}, {dom['dxt']}>")
+    gbs = dom["bytes"] / (dom["ms"] * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kname,
+                "kernel_ms": round(dom["ms"], 4), "algorithmic_bytes": dom["bytes"]}
+    class_summary = [dict(kernel=("normal" if c["normal"] else "fast"), cd=names[c["cd"]], dxt=c["dxt"],
+                          jobs=c["njobs"], texels=c["texels"], ms=round(c["ms"], 4),
+                          gbs=round(c["bytes"] / (c["ms"] * 1e-3) / 1e9, 1)) for c in classes]
+
+    plain = None
+    if not a.no_plain:
+        s.encode_jobs_plain(imgs)
+        s.synchronize()
+        k2 = max(1, min(a.steps, 3))
+        t0 = time.perf_counter()
+        for _ in range(k2):
+            s.encode_jobs_plain(imgs)
+        s.synchronize()
+        pdt = (time.perf_counter() - t0) / k2
+        plain = dict(ms_per_step=round(pdt * 1e3, 3), value=round(texels / pdt / 1e6, 1),
+                     launches=len(items), speedup_batched=round(pdt / dt, 2))
+
+    cpu = None
+    if not a.no_cpu:
+        cpu = cpu_baseline(items)
+        if cpu:
+            cpu["value"] = round(cpu["value"], 2)
+
+    out = {
+        "metric": "Mtexels/s bit-exact, batched persistent vs per-texture plain; p99 serve latency",
+        "value": round(value, 1), "unit": "Mtexels/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": round(dt * 1e3, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u8", "data": "synthetic",
+        "config": {"workload": a.workload if a.workload != "batch" else
+                   "batch: 2000 RGBA8 textures 4..2048 per side + full mip chains (16467 images), "
+                   "DXT1/3/5 30/30/40%, 8 distance modes, quick selection, REFINE_ALWAYS, DITHER_NONE",
+                   "images": len(items), "texels": texels, "schedule": {"wg_per_cu": sched[0], "streams": sched[1]}},
+        "plain_per_texture": plain,
+        "roofline": roofline,
+        "classes": class_summary,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    batch.destroy()
+    imgs.free()
+
+
+if __name__ == "__main__":
+    main()

@@ -58,6 +58,10 @@ S2TC_API int s2tc_amd_memcpy_h2d(void *dst, const void *src, size_t bytes);
 S2TC_API int s2tc_amd_memcpy_d2h(void *dst, const void *src, size_t bytes);
 S2TC_API int s2tc_amd_memset(void *dst, int value, size_t bytes);
 S2TC_API int s2tc_amd_synchronize(void);
+/* Device-to-device copy by a plain streaming kernel that reads and writes
+ * each byte once (16-byte aligned pointers and size); a known-size workload
+ * for calibrating hardware byte counters. */
+S2TC_API int s2tc_amd_memcpy_d2d_kernel(void *dst, const void *src, size_t bytes, void *stream);
 
 /* Encode one image already resident on the device.  d_src: width*height
  * texels of 4 bytes (row stride = width); d_dst: ceil(w/4)*ceil(h/4) blocks
@@ -65,6 +69,53 @@ S2TC_API int s2tc_amd_synchronize(void);
  * hipStream_t, NULL = default stream). */
 S2TC_API int s2tc_amd_encode_image(const void *d_src, int src_kind, int width, int height, void *d_dst,
                           int dxt, int cd, int nrandom, int refine, uint64_t seed, void *stream);
+
+/* ---- batches: many images (textures x mip levels) in one graph launch ---- */
+
+typedef struct {
+	const void *src;   /* device: width*height texels, 4 bytes each (see src_kind) */
+	void *dst;         /* device: ceil(width/4)*ceil(height/4) blocks in raster order */
+	int width, height;
+	int src_kind;      /* S2TC_SRC_QUANTIZED or S2TC_SRC_RGBA8 */
+	int dxt, cd, nrandom, refine;
+	uint64_t seed;     /* generator key of this image's random candidates */
+} s2tc_amd_job;
+
+/* per-kernel-class timing of a batch (s2tc_amd_batch_class_times) */
+typedef struct {
+	int normal;        /* 0: quick-mode class, 1: MODE_NORMAL class */
+	int cd, dxt;       /* dxt is -1 for MODE_NORMAL classes (mixed formats) */
+	int njobs;
+	long long blocks, texels;
+	long long bytes;   /* algorithmic HBM bytes: 4 per texel read + block bytes written */
+	double ms;         /* kernel time of one launch */
+} s2tc_amd_class_time;
+
+typedef struct s2tc_amd_batch s2tc_amd_batch;
+
+/* Plans a batch: jobs are grouped into kernel classes (quick mode per
+ * distance and format, MODE_NORMAL per distance); each class runs one
+ * persistent kernel over a packed list of 4x4 blocks, and all classes are
+ * captured into one hipGraph.  The job array is copied; the device buffers
+ * it points to must stay valid while the batch is used. */
+S2TC_API int s2tc_amd_batch_create(const s2tc_amd_job *jobs, int njobs, s2tc_amd_batch **out);
+/* Schedule: workgroups per CU of each persistent kernel (0 = one workgroup
+ * per tile, i.e. not persistent) and the number of streams the classes are
+ * spread over inside the graph.  Re-captures the graph. */
+S2TC_API int s2tc_amd_batch_set_schedule(s2tc_amd_batch *b, int wg_per_cu, int streams);
+/* Times every schedule of a small grid (reps graph launches each) and keeps
+ * the fastest; *best_ms gets its time per launch. */
+S2TC_API int s2tc_amd_batch_autotune(s2tc_amd_batch *b, void *stream, int reps, double *best_ms);
+S2TC_API int s2tc_amd_batch_get_schedule(const s2tc_amd_batch *b, int *wg_per_cu, int *streams);
+/* One launch of the whole batch (graph replay), asynchronous on stream. */
+S2TC_API int s2tc_amd_batch_launch(s2tc_amd_batch *b, void *stream);
+/* Launches each class alone, timed with HIP events; returns the class count
+ * (at most max_out entries written). */
+S2TC_API int s2tc_amd_batch_class_times(s2tc_amd_batch *b, void *stream, s2tc_amd_class_time *out, int max_out);
+S2TC_API int s2tc_amd_batch_destroy(s2tc_amd_batch *b);
+
+/* The same jobs with one plain kernel launch per job (the per-texture path). */
+S2TC_API int s2tc_amd_encode_jobs_plain(const s2tc_amd_job *jobs, int njobs, void *stream);
 
 #ifdef __cplusplus
 }

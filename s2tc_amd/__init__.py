@@ -131,3 +131,130 @@ def encode_image(texels, width, height, dxt, cd=WAVG, nrandom=-1, refine=REFINE_
     src.free()
     dst.free()
     return out
+
+
+# ---- batches ----
+
+class Job(ctypes.Structure):
+    """s2tc_amd_job"""
+    _fields_ = [("src", _P), ("dst", _P), ("width", _I), ("height", _I), ("src_kind", _I), ("dxt", _I),
+                ("cd", _I), ("nrandom", _I), ("refine", _I), ("seed", _U64)]
+
+
+class ClassTime(ctypes.Structure):
+    """s2tc_amd_class_time"""
+    _fields_ = [("normal", _I), ("cd", _I), ("dxt", _I), ("njobs", _I), ("blocks", ctypes.c_longlong),
+                ("texels", ctypes.c_longlong), ("bytes", ctypes.c_longlong), ("ms", ctypes.c_double)]
+
+
+def _bind_batch(L):
+    if getattr(L, "_batch_bound", False):
+        return L
+    L.s2tc_amd_batch_create.argtypes = [_P, _I, ctypes.POINTER(_P)]
+    L.s2tc_amd_batch_set_schedule.argtypes = [_P, _I, _I]
+    L.s2tc_amd_batch_get_schedule.argtypes = [_P, ctypes.POINTER(_I), ctypes.POINTER(_I)]
+    L.s2tc_amd_batch_autotune.argtypes = [_P, _P, _I, ctypes.POINTER(ctypes.c_double)]
+    L.s2tc_amd_batch_launch.argtypes = [_P, _P]
+    L.s2tc_amd_batch_class_times.argtypes = [_P, _P, _P, _I]
+    L.s2tc_amd_batch_destroy.argtypes = [_P]
+    L.s2tc_amd_encode_jobs_plain.argtypes = [_P, _I, _P]
+    L._batch_bound = True
+    return L
+
+
+class DeviceImages:
+    """A set of images packed into one device buffer, with one output buffer
+    for all their blocks.  items: iterable of dicts with rgba (uint8 array of
+    width*height*4), width, height, dxt, cd, nrandom, refine, seed."""
+
+    ALIGN = 256
+
+    def __init__(self, items, src_kind=SRC_RGBA8):
+        items = list(items)
+        self.items = items
+        so, do = [], []
+        s_total = d_total = 0
+        for it in items:
+            so.append(s_total)
+            s_total += (it["width"] * it["height"] * 4 + self.ALIGN - 1) // self.ALIGN * self.ALIGN
+            do.append(d_total)
+            nb = nblocks(it["width"], it["height"]) * block_bytes(it["dxt"])
+            d_total += (nb + self.ALIGN - 1) // self.ALIGN * self.ALIGN
+        self.src = DeviceBuffer(max(s_total, 1))
+        self.dst = DeviceBuffer(max(d_total, 1))
+        self.src_off, self.dst_off = so, do
+        self.texels = sum(it["width"] * it["height"] for it in items)
+        self.bytes_in = self.texels * 4
+        self.bytes_out = sum(nblocks(it["width"], it["height"]) * block_bytes(it["dxt"]) for it in items)
+        staging = np.zeros(s_total, np.uint8)
+        for it, o in zip(items, so):
+            a = np.ascontiguousarray(it["rgba"], dtype=np.uint8).reshape(-1)
+            staging[o:o + a.size] = a
+        check(lib().s2tc_amd_memcpy_h2d(self.src.ptr, staging.ctypes.data_as(_P), s_total))
+        del staging
+        self.jobs = (Job * len(items))()
+        for i, it in enumerate(items):
+            self.jobs[i] = Job(self.src.ptr + so[i], self.dst.ptr + do[i], it["width"], it["height"], src_kind,
+                               it["dxt"], it["cd"], it["nrandom"], it["refine"], it["seed"])
+
+    def output(self, i, host=None):
+        """Blocks of image i (copies the whole output buffer once if host is None)."""
+        it = self.items[i]
+        n = nblocks(it["width"], it["height"]) * block_bytes(it["dxt"])
+        host = self.dst.to_host() if host is None else host
+        return host[self.dst_off[i]:self.dst_off[i] + n]
+
+    def free(self):
+        self.src.free()
+        self.dst.free()
+
+
+class Batch:
+    """s2tc_amd_batch: the images of a DeviceImages in one graph launch."""
+
+    def __init__(self, images):
+        L = _bind_batch(lib())
+        self.images = images
+        h = _P()
+        check(L.s2tc_amd_batch_create(ctypes.cast(images.jobs, _P), len(images.jobs), ctypes.byref(h)))
+        self.h = h.value
+
+    def launch(self, stream=None):
+        check(lib().s2tc_amd_batch_launch(self.h, stream))
+
+    def set_schedule(self, wg_per_cu, streams):
+        check(lib().s2tc_amd_batch_set_schedule(self.h, wg_per_cu, streams))
+
+    def schedule(self):
+        a, b = _I(), _I()
+        check(lib().s2tc_amd_batch_get_schedule(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def autotune(self, reps=3, stream=None):
+        ms = ctypes.c_double()
+        check(lib().s2tc_amd_batch_autotune(self.h, stream, reps, ctypes.byref(ms)))
+        return ms.value
+
+    def class_times(self, stream=None):
+        arr = (ClassTime * 64)()
+        n = lib().s2tc_amd_batch_class_times(self.h, stream, ctypes.cast(arr, _P), 64)
+        if n < 0:
+            check(n)
+        return [dict(normal=c.normal, cd=c.cd, dxt=c.dxt, njobs=c.njobs, blocks=c.blocks, texels=c.texels,
+                     bytes=c.bytes, ms=c.ms) for c in arr[:n]]
+
+    def destroy(self):
+        if self.h:
+            lib().s2tc_amd_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+def encode_jobs_plain(images, stream=None):
+    L = _bind_batch(lib())
+    check(L.s2tc_amd_encode_jobs_plain(ctypes.cast(images.jobs, _P), len(images.jobs), stream))

@@ -411,8 +411,11 @@ S2TC_D void encode_block_normal(uint32_t tex, int t, int w, int h, int dxt, int 
 	const int ci = min((int)(cbest >> 10), m - 1), cj = min((int)(cbest & 1023u), m - 1);
 	uint32_t c0 = sm.cand[ci] & 0xffffu, c1 = sm.cand[cj] & 0xffffu;
 	int a0 = 0, a1 = 0;
-	if (dxt == DXT5) {  // reduce_colors_inplace_2fixpoints (:1006, :415-478)
-		__syncthreads();
+	// reduce_colors_inplace_2fixpoints for DXT5 alpha (:1006, :415-478).  The
+	// barriers are outside the format test: the four blocks of a workgroup
+	// may have different formats in the batch kernels.
+	__syncthreads();
+	if (dxt == DXT5) {
 		for (int p = 0; p < pbw; ++p) {
 			const int i = 16 * p + t;
 			const bool irow = i < m;
@@ -430,7 +433,9 @@ S2TC_D void encode_block_normal(uint32_t tex, int t, int w, int h, int dxt, int 
 #pragma unroll
 			for (int k = 0; k < 4; ++k) rp[k] = make_int4(dv[4 * k], dv[4 * k + 1], dv[4 * k + 2], dv[4 * k + 3]);
 		}
-		__syncthreads();
+	}
+	__syncthreads();
+	if (dxt == DXT5) {
 		const uint32_t abest = reduce_rows<false>(sm.rows, t, m, pbw);
 		const int ai = min((int)(abest >> 10), m - 1), aj = min((int)(abest & 1023u), m - 1);
 		a0 = (int)(sm.cand[ai] >> 16);

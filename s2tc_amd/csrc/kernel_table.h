@@ -1,10 +1,29 @@
 // kernel_table.h - per-distance-mode table of encoder kernels (host side).
 #pragma once
 
+#include <cstdint>
+
+// One image (a texture's mip level) of a batch, as the persistent kernels see it.
+struct JobDesc {
+	const uint32_t *src;  // width*height texels, 4 bytes each
+	uint8_t *dst;         // blocks in raster order
+	int width, height;
+	int dxt, nrandom, refine, raw;  // raw: RGBA8 quantized on load with DITHER_NONE
+	unsigned long long seed;
+	long long block0;     // first block of this job in its class's packed block index space
+};
+
+// First and last job index a batch tile touches.
+struct TileJobs {
+	int lo, hi;
+};
+
 struct KernelTable {
 	const void *fast[3][2];       // [dxt][raw]: k_fast<CD, dxt, raw>, null for NORMALMAP
 	const void *normal[2];        // [raw]: k_normal<CD, raw>
 	int normal_smem_per_cand;     // LDS bytes per candidate slot for a 4-block workgroup
+	const void *fast_batch[3];    // [dxt]: persistent quick-mode kernel over a job list
+	const void *normal_batch;     // persistent MODE_NORMAL kernel over a job list
 };
 
 extern "C" {

@@ -110,6 +110,84 @@ __global__ __launch_bounds__(64) void k_normal(const uint32_t *__restrict__ src,
 	}
 }
 
+// ---- persistent batch kernels ----
+//
+// Jobs of one class (same distance; same format for quick mode) share a
+// packed block index space: job j owns blocks [block0_j, block0_{j+1}).
+// Tiles are TILE consecutive packed blocks; tile_jobs[t] = (first, last) job
+// index the tile touches, so each lane or group finds its job by a short
+// binary search.  Workgroups stride over the tiles (tile = blockIdx.x + k *
+// gridDim.x), so the grid size is a free parameter and no counters need
+// resetting between launches.
+
+S2TC_D int find_job(const JobDesc *__restrict__ jobs, int lo, int hi, long long gb)
+{
+	while (lo < hi) {
+		const int mid = (lo + hi + 1) >> 1;
+		if (jobs[mid].block0 <= gb) lo = mid;
+		else hi = mid - 1;
+	}
+	return lo;
+}
+
+template <int CD, int DXT>
+__global__ __launch_bounds__(256) void k_fast_batch(const JobDesc *__restrict__ jobs, const TileJobs *__restrict__ tile_jobs,
+                                                    long long nblocks, int ntiles)
+{
+	for (int tile = (int)blockIdx.x; tile < ntiles; tile += (int)gridDim.x) {
+		const long long gb = (long long)tile * 256 + threadIdx.x;
+		if (gb >= nblocks) continue;
+		const TileJobs tj = tile_jobs[tile];
+		const JobDesc &j = jobs[find_job(jobs, tj.lo, tj.hi, gb)];
+		const int b = (int)(gb - j.block0);
+		const int bw = (j.width + 3) >> 2;
+		const int by = b / bw, bx = b - by * bw;
+		const int w = min(4, j.width - 4 * bx), h = min(4, j.height - 4 * by);
+		const bool vec = (j.width & 3) == 0;
+		uint32_t tx[16];
+		if (j.raw) load_block<DXT, true>(j.src, j.width, bx, by, w, h, vec, tx);
+		else load_block<DXT, false>(j.src, j.width, bx, by, w, h, vec, tx);
+		const BlockOut o = encode_block_fast<CD, DXT>(tx, w, h, j.refine);
+		store_block<DXT>(j.dst, (size_t)b, o);
+	}
+}
+
+template <int CD>
+__global__ __launch_bounds__(64) void k_normal_batch(const JobDesc *__restrict__ jobs, const TileJobs *__restrict__ tile_jobs,
+                                                     long long nblocks, int ntiles, int pbw_max)
+{
+	extern __shared__ __align__(16) uint8_t smem[];
+	const int lane = (int)threadIdx.x;
+	const int g = lane >> 4, t = lane & 15;
+	const GroupSmem<CD> sm = group_smem<CD>(smem, g, 16 * pbw_max);
+	for (int tile = (int)blockIdx.x; tile < ntiles; tile += (int)gridDim.x) {
+		long long gb = (long long)tile * 4 + g;
+		const bool have = gb < nblocks;
+		if (!have) gb = nblocks - 1;
+		const TileJobs tj = tile_jobs[tile];
+		const JobDesc &j = jobs[find_job(jobs, tj.lo, tj.hi, gb)];
+		const int b = (int)(gb - j.block0);
+		const int bw = (j.width + 3) >> 2;
+		const int by = b / bw, bx = b - by * bw;
+		const int w = min(4, j.width - 4 * bx), h = min(4, j.height - 4 * by);
+		const int x = t >> 2, y = t & 3;
+		uint32_t tex = 0;
+		if (x < w && y < h) {
+			tex = j.src[(size_t)(4 * by + y) * (size_t)j.width + 4 * bx + x];
+			if (j.raw) tex = quant_none_rt(tex, j.dxt);
+		}
+		const uint32_t bkey = s2tc_prng_block_key((uint64_t)j.seed, (uint32_t)b);
+		const int pbw = (16 + (j.nrandom > 0 ? j.nrandom : 0) + 15) / 16;
+		BlockOut o;
+		encode_block_normal<CD>(tex, t, w, h, j.dxt, j.nrandom, j.refine, bkey, pbw, sm, o);
+		if (have && t == 0) {
+			if (j.dxt == DXT1) store_block<DXT1>(j.dst, (size_t)b, o);
+			else store_block<DXT5>(j.dst, (size_t)b, o);
+		}
+		__syncthreads();
+	}
+}
+
 }  // namespace s2tc
 
 using namespace s2tc;
@@ -129,6 +207,14 @@ extern "C" void S2TC_CAT(s2tc_kernels_cd, S2TC_CD)(KernelTable *kt)
 #else
 	for (int d = 0; d < 3; ++d) kt->fast[d][0] = kt->fast[d][1] = nullptr;
 #endif
+#if S2TC_CD != 7
+	kt->fast_batch[DXT1] = (const void *)k_fast_batch<S2TC_CD, DXT1>;
+	kt->fast_batch[DXT3] = (const void *)k_fast_batch<S2TC_CD, DXT3>;
+	kt->fast_batch[DXT5] = (const void *)k_fast_batch<S2TC_CD, DXT5>;
+#else
+	for (int d = 0; d < 3; ++d) kt->fast_batch[d] = nullptr;
+#endif
+	kt->normal_batch = (const void *)k_normal_batch<S2TC_CD>;
 	kt->normal[0] = (const void *)k_normal<S2TC_CD, false>;
 	kt->normal[1] = (const void *)k_normal<S2TC_CD, true>;
 	kt->normal_smem_per_cand = (int)(4 * (sizeof(uint32_t) + sizeof(Dist<S2TC_CD>::F) + S2TC_RS * sizeof(int)));

@@ -1,0 +1,87 @@
+"""The persistent batch (one graph launch) against the oracle, the plain
+per-texture launches and the compiled reference's digest of the full batch."""
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DIGESTS = json.load(open(os.path.join(ROOT, "tests", "golden", "digests.json")))
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def small_batch_items(seed, n, max_k=7, serve_mix=True):
+    from s2tc_amd import synth, workloads
+    specs = synth.batch_specs(n * 3, seed, serve_mix)
+    specs = [s for s in specs if s.width <= (1 << max_k) and s.height <= (1 << max_k)][:n]
+    items = []
+    for s in specs:
+        base = synth.texture(s)
+        for li, (w, h, lv) in enumerate(synth.mip_chain(base, s.width, s.height)):
+            items.append(dict(rgba=lv, width=w, height=h, dxt=s.dxt, cd=s.cd, nrandom=s.nrandom,
+                              refine=s.refine, seed=workloads.level_seed(s.seed, li)))
+    return items
+
+
+def test_batch_equals_oracle_and_plain(gpu, oracle):
+    items = small_batch_items(7, 60)
+    # extra odd sizes and every nrandom class
+    rng = np.random.default_rng(1)
+    for (w, h, dxt, cd, nr, rf) in [(5, 3, 0, 5, -1, 1), (9, 13, 1, 2, 0, 2), (7, 7, 2, 7, 16, 2),
+                                    (33, 6, 2, 3, 64, 2), (1, 1, 2, 0, 5, 0), (3, 1, 0, 7, -1, 1)]:
+        items.append(dict(rgba=rng.integers(0, 256, w * h * 4).astype(np.uint8), width=w, height=h, dxt=dxt,
+                          cd=cd, nrandom=nr, refine=rf, seed=int(rng.integers(0, 2**63))))
+    _say(f"batch of {len(items)} images")
+    imgs = gpu.DeviceImages(items)
+    b = gpu.Batch(imgs)
+    for sched in [(4, 4), (0, 1), (1, 8)]:
+        imgs.dst.fill(0)
+        b.set_schedule(*sched)
+        _say(f"batch launch, schedule {sched}")
+        b.launch()
+        gpu.synchronize()
+        host = imgs.dst.to_host()
+        for i, it in enumerate(items):
+            q = oracle.rgb565_image(it["rgba"], it["width"], it["height"], 4, {0: 1, 1: 4, 2: 8}[it["dxt"]], 0)
+            exp = oracle.encode_quantized(q, it["width"], it["height"], it["dxt"], it["cd"], it["nrandom"],
+                                          it["refine"], it["seed"])
+            assert np.array_equal(imgs.output(i, host), exp), (sched, i, {k: v for k, v in it.items() if k != "rgba"})
+    imgs.dst.fill(0)
+    _say("plain per-image launches")
+    gpu.encode_jobs_plain(imgs)
+    gpu.synchronize()
+    plain = imgs.dst.to_host()
+    assert np.array_equal(plain, host)
+    b.destroy()
+    imgs.free()
+
+
+def test_full_batch_matches_reference_digest(gpu):
+    """The 2000-texture batch with all mip levels, one graph launch, against
+    the compiled reference's per-job digests."""
+    from s2tc_amd import workloads
+    exp = DIGESTS["workloads"]["batch"]
+    _say("building the 2000-texture batch")
+    jobs = workloads.batch_jobs(workloads.BATCH_SEED, workloads.BATCH_COUNT)
+    items = [dict(rgba=lv["rgba"], width=lv["width"], height=lv["height"], dxt=t["dxt"], cd=t["cd"],
+                  nrandom=t["nrandom"], refine=t["refine"], seed=lv["seed"]) for _, _, t, lv in jobs.jobs()]
+    assert len(items) == exp["jobs"] and jobs.texels == exp["texels"]
+    imgs = gpu.DeviceImages(items)
+    del jobs
+    b = gpu.Batch(imgs)
+    _say("full batch launch")
+    b.launch()
+    gpu.synchronize()
+    host = imgs.dst.to_host()
+    hashes = b"".join(hashlib.sha256(imgs.output(i, host).tobytes()).digest() for i in range(len(items)))
+    assert hashlib.sha256(hashes).hexdigest() == exp["sha256_of_job_sha256s"]
+    b.destroy()
+    imgs.free()
