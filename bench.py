@@ -166,12 +166,13 @@ def main():
                 c["ms"] = min(c["ms"], n["ms"])
     dom = max(classes, key=lambda c: c["ms"])
     names = s.COLORDIST_NAMES
-    kname = (f"k_normal_batch<{dom['cd']}>" if dom["normal"] else f"k_fast_batch<{dom['cd']}, {dom['dxt']}>")
+    kname = {0: f"k_fast_batch<{dom['cd']}, {dom['dxt']}>", 1: f"k_normal_batch<{dom['cd']}>",
+             2: f"k_all16_batch<{dom['cd']}, {dom['dxt']}>"}[dom["normal"]]
     gbs = dom["bytes"] / (dom["ms"] * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kname,
                 "kernel_ms": round(dom["ms"], 4), "algorithmic_bytes": dom["bytes"]}
-    class_summary = [dict(kernel=("normal" if c["normal"] else "fast"), cd=names[c["cd"]], dxt=c["dxt"],
+    class_summary = [dict(kernel={0: "fast", 1: "normal", 2: "all16"}[c["normal"]], cd=names[c["cd"]], dxt=c["dxt"],
                           jobs=c["njobs"], texels=c["texels"], ms=round(c["ms"], 4),
                           gbs=round(c["bytes"] / (c["ms"] * 1e-3) / 1e9, 1)) for c in classes]
 

@@ -83,8 +83,9 @@ typedef struct {
 
 /* per-kernel-class timing of a batch (s2tc_amd_batch_class_times) */
 typedef struct {
-	int normal;        /* 0: quick-mode class, 1: MODE_NORMAL class */
-	int cd, dxt;       /* dxt is -1 for MODE_NORMAL classes (mixed formats) */
+	int normal;        /* 0: quick mode, 1: MODE_NORMAL with random colours (16 lanes
+	                      per block), 2: MODE_NORMAL with the block colours only */
+	int cd, dxt;       /* dxt is -1 for kind 1 classes (mixed formats) */
 	int njobs;
 	long long blocks, texels;
 	long long bytes;   /* algorithmic HBM bytes: 4 per texel read + block bytes written */

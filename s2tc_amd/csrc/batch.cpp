@@ -2,9 +2,11 @@
 //
 // Plan: jobs are grouped into kernel classes.  Quick-mode jobs (nrandom < 0,
 // distance != NORMALMAP) form one class per (distance, format) and run
-// k_fast_batch (one lane per 4x4 block, tiles of 256 blocks); all other jobs
-// form one class per distance and run k_normal_batch (16 lanes per block,
-// tiles of 4 blocks).  Within a class the blocks of all its jobs are packed
+// k_fast_batch (one lane per 4x4 block, tiles of 256 blocks); MODE_NORMAL
+// jobs without random colours form one class per (distance, format) and run
+// k_all16_batch (also one lane per block); jobs with random colours form one
+// class per distance and run k_normal_batch (16 lanes per block, tiles of 4
+// blocks).  Within a class the blocks of all its jobs are packed
 // into one index space, so a tile may cover many small mip levels and no
 // lanes idle at job boundaries.  Each class kernel is persistent: a grid of
 // wg_per_cu * CUs workgroups strides over the tiles.  The class launches are
@@ -21,7 +23,8 @@
 namespace s2tc {
 
 struct ClassPlan {
-	bool normal = false;
+	bool normal = false;         // 16-lanes-per-block kernel
+	bool all16 = false;          // one-lane-per-block MODE_NORMAL kernel
 	int cd = 0, dxt = -1;
 	std::vector<JobDesc> jobs;
 	std::vector<int> job_index;  // index into the caller's job array
@@ -63,7 +66,8 @@ static int launch_class(const ClassPlan &c, int wg_per_cu, int ncu, hipStream_t 
 		return hip_check(hipLaunchKernel(c.kernel, dim3(grid), dim3(64), args, c.smem, st), "launch k_normal_batch");
 	}
 	void *args[] = {(void *)&jobs, (void *)&tiles, (void *)&nb, (void *)&ntiles};
-	return hip_check(hipLaunchKernel(c.kernel, dim3(grid), dim3(256), args, 0, st), "launch k_fast_batch");
+	return hip_check(hipLaunchKernel(c.kernel, dim3(grid), dim3(256), args, 0, st),
+	                 c.all16 ? "launch k_all16_batch" : "launch k_fast_batch");
 }
 
 static void destroy_graph(s2tc_amd_batch *b)
@@ -135,8 +139,9 @@ int s2tc_amd_batch_create(const s2tc_amd_job *jobs, int njobs, s2tc_amd_batch **
 	hipDeviceProp_t prop;
 	if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
 		b->ncu = prop.multiProcessorCount;
-	// classes: index 0..23 quick (cd*3 + dxt), 24..31 MODE_NORMAL (cd)
-	std::vector<int> slot(32, -1);
+	// classes: 0..23 quick (cd*3 + dxt), 24..31 MODE_NORMAL with random
+	// colours (cd), 32..55 MODE_NORMAL without (32 + cd*3 + dxt)
+	std::vector<int> slot(56, -1);
 	for (int i = 0; i < njobs; ++i) {
 		const s2tc_amd_job &j = jobs[i];
 		int dxt = j.dxt, cd = j.cd, refine = j.refine;
@@ -155,15 +160,17 @@ int s2tc_amd_batch_create(const s2tc_amd_job *jobs, int njobs, s2tc_amd_batch **
 			return set_error(S2TC_E_ARG, "job %d: bad source kind", i);
 		}
 		const bool fast = is_fast(cd, j.nrandom);
-		const int key = fast ? cd * 3 + dxt : 24 + cd;
+		const bool all16 = !fast && j.nrandom <= 0;
+		const int key = fast ? cd * 3 + dxt : all16 ? 32 + cd * 3 + dxt : 24 + cd;
 		if (slot[key] < 0) {
 			slot[key] = (int)b->classes.size();
 			b->classes.emplace_back();
 			ClassPlan &c = b->classes.back();
-			c.normal = !fast;
+			c.normal = !fast && !all16;
+			c.all16 = all16;
 			c.cd = cd;
-			c.dxt = fast ? dxt : -1;
-			c.tile = fast ? 256 : 4;
+			c.dxt = c.normal ? -1 : dxt;
+			c.tile = c.normal ? 4 : 256;
 		}
 		ClassPlan &c = b->classes[slot[key]];
 		JobDesc d;
@@ -186,12 +193,13 @@ int s2tc_amd_batch_create(const s2tc_amd_job *jobs, int njobs, s2tc_amd_batch **
 		c.job_index.push_back(i);
 		// rough cost: quick mode ~1 per block, MODE_NORMAL ~ candidate pairs
 		const int m = 16 + std::max(0, d.nrandom);
-		c.cost += fast ? (double)nb : (double)nb * (m * (m - 1) / 2) * (dxt == DXT5 ? 2 : 1) / 8.0;
+		c.cost += fast ? (double)nb : all16 ? (double)nb * 8.0 * (dxt == DXT5 ? 2 : 1)
+		                                    : (double)nb * (m * (m - 1) / 2) * (dxt == DXT5 ? 2 : 1) / 8.0;
 	}
 	int rc = 0;
 	for (auto &c : b->classes) {
 		const KernelTable &kt = kernel_table(c.cd);
-		c.kernel = c.normal ? kt.normal_batch : kt.fast_batch[c.dxt];
+		c.kernel = c.normal ? kt.normal_batch : c.all16 ? kt.all16_batch[c.dxt] : kt.fast_batch[c.dxt];
 		if (!c.kernel) {
 			rc = set_error(S2TC_E_ARG, "no batch kernel for distance %d", c.cd);
 			break;
@@ -338,7 +346,7 @@ int s2tc_amd_batch_class_times(s2tc_amd_batch *b, void *stream, s2tc_amd_class_t
 		(void)hipEventElapsedTime(&ms, e0, e1);
 		if (out && n < max_out) {
 			s2tc_amd_class_time &o = out[n];
-			o.normal = c.normal ? 1 : 0;
+			o.normal = c.normal ? 1 : c.all16 ? 2 : 0;
 			o.cd = c.cd;
 			o.dxt = c.dxt;
 			o.njobs = (int)c.jobs.size();

@@ -21,15 +21,42 @@ namespace s2tc {
 
 struct BlockOut { uint32_t w[4]; };
 
+// Per-block lane masks: bit k (texel x = k >> 2, y = k & 3) of `valid` is
+// set if the texel is inside the block, bit k of `trans` if it is a DXT1
+// transparent texel (alpha bit 0) inside the block.
+struct Masks {
+	uint32_t valid, trans;
+};
+
+template <int DXT>
+S2TC_D Masks block_masks(const uint32_t (&tx)[16], int w, int h)
+{
+	Masks m{0u, 0u};
+#pragma unroll
+	for (int k = 0; k < 16; ++k) {
+		const int x = k >> 2, y = k & 3;
+		const bool v = x < w && y < h;
+		m.valid |= (v ? 1u : 0u) << k;
+		if (DXT == DXT1) m.trans |= ((v && texel_a(tx[k]) == 0) ? 1u : 0u) << k;
+	}
+	return m;
+}
+
+// packed cluster sum term of a texel: r | g << 10 | b << 21
+S2TC_D uint32_t pack_sum(uint32_t t) { return (t & 0x1fu) | ((t & 0x3f00u) << 2) | ((t & 0x1f0000u) << 5); }
+
 template <int CD, bool TRANS>
 struct ColorRefineLane {
 	typedef Dist<CD> D;
 	typedef typename D::F F;
-	// one s2tc_try_encode_block pass (:582-643) over the lane's texels
-	static S2TC_D uint32_t try_pass(const uint32_t (&tx)[16], const F (&f)[16], int w, int h, uint32_t c0,
+	// one s2tc_try_encode_block pass (:582-643) over the lane's texels,
+	// without branches: texels outside the block get index 0 and do not
+	// count, transparent DXT1 texels get index 3 and do not count.
+	template <bool FULL>
+	static S2TC_D uint32_t try_pass(const uint32_t (&tx)[16], const F (&f)[16], const Masks &mk, uint32_t c0,
 	                                uint32_t c1, uint32_t &score, int &n1, uint32_t &s1)
 	{
-		F e0 = D::feat(c0), e1 = D::feat(c1);
+		const F e0 = D::feat(c0), e1 = D::feat(c1);
 		uint32_t bits = 0;
 		score = 0;
 		n1 = 0;
@@ -37,20 +64,17 @@ struct ColorRefineLane {
 #pragma unroll
 		for (int k = 0; k < 16; ++k) {
 			const int x = k >> 2, y = k & 3, slot = y * 4 + x;
-			const bool valid = x < w && y < h;
-			if (!valid) continue;
-			if (TRANS && texel_a(tx[k]) == 0) {
-				bits |= 3u << (2 * slot);
-				continue;
-			}
-			int d0 = D::d(f[k], e0), d1 = D::d(f[k], e1);
-			bool b = d1 < d0;
-			score += (uint32_t)(b ? d1 : d0);
-			uint32_t v = texel565(tx[k]);
-			uint32_t pk = (v >> 11) | (((v >> 5) & 63u) << 10) | ((v & 31u) << 21);
-			n1 += b ? 1 : 0;
-			s1 += b ? pk : 0u;
-			bits |= (b ? 1u : 0u) << (2 * slot);
+			const bool valid = FULL || ((mk.valid >> k) & 1u);
+			const bool transp = TRANS && ((mk.trans >> k) & 1u);
+			const bool use = valid && !transp;
+			const int d0 = D::d(f[k], e0), d1 = D::d(f[k], e1);
+			const bool b = d1 < d0;
+			score += use ? (uint32_t)(b ? d1 : d0) : 0u;
+			const bool b1 = use && b;
+			n1 += b1 ? 1 : 0;
+			s1 += b1 ? pack_sum(tx[k]) : 0u;
+			const uint32_t idx = transp ? 3u : (b1 ? 1u : 0u);
+			bits |= idx << (2 * slot);
 		}
 		return bits;
 	}
@@ -90,8 +114,9 @@ S2TC_D void color_fix_order(uint32_t &bits, uint32_t &c0, uint32_t &c1)
 	}
 }
 
-// alpha pass of s2tc_try_encode_block with have_0_255 (:582-643)
-S2TC_D uint64_t alpha_try_lane(const uint32_t (&tx)[16], int w, int h, int a0, int a1, uint32_t &score, int &n0,
+// alpha pass of s2tc_try_encode_block with have_0_255 (:582-643), branch-free
+template <bool FULL>
+S2TC_D uint64_t alpha_try_lane(const uint32_t (&tx)[16], const Masks &mk, int a0, int a1, uint32_t &score, int &n0,
                                int &s0, int &n1, int &s1)
 {
 	uint64_t bits = 0;
@@ -100,28 +125,21 @@ S2TC_D uint64_t alpha_try_lane(const uint32_t (&tx)[16], int w, int h, int a0, i
 #pragma unroll
 	for (int k = 0; k < 16; ++k) {
 		const int x = k >> 2, y = k & 3, slot = y * 4 + x;
-		if (!(x < w && y < h)) continue;
-		int a = texel_a(tx[k]);
-		int d0 = adist(a, a0), d1 = adist(a, a1);
-		bool b = d1 < d0;
-		int bd = b ? d1 : d0;
-		int dz = a * a, df = (255 - a) * (255 - a);
-		uint32_t idx;
-		if (dz <= bd) {
-			idx = 6;
-			score += (uint32_t)dz;
-		} else if (df <= bd) {
-			idx = 7;
-			score += (uint32_t)df;
-		} else {
-			idx = b ? 1u : 0u;
-			score += (uint32_t)bd;
-			n1 += b ? 1 : 0;
-			s1 += b ? a : 0;
-			n0 += b ? 0 : 1;
-			s0 += b ? 0 : a;
-		}
-		bits |= (uint64_t)idx << (3 * slot);
+		const bool valid = FULL || ((mk.valid >> k) & 1u);
+		const int a = texel_a(tx[k]);
+		const int d0 = adist(a, a0), d1 = adist(a, a1);
+		const bool b = d1 < d0;
+		const int bd = b ? d1 : d0;
+		const int dz = a * a, df = (255 - a) * (255 - a);
+		const bool z = dz <= bd, full = !z && df <= bd;
+		const bool acc = valid && !z && !full;
+		const uint32_t idx = z ? 6u : full ? 7u : (b ? 1u : 0u);
+		score += valid ? (uint32_t)(z ? dz : full ? df : bd) : 0u;
+		n1 += (acc && b) ? 1 : 0;
+		s1 += (acc && b) ? a : 0;
+		n0 += (acc && !b) ? 1 : 0;
+		s0 += (acc && !b) ? a : 0;
+		bits |= (uint64_t)(valid ? idx : 0u) << (3 * slot);
 	}
 	return bits;
 }
@@ -150,7 +168,8 @@ S2TC_D void alpha_fix_order(uint64_t &bits, int &a0, int &a1)
 }
 
 // DXT5 alpha refinement NEVER/ALWAYS/LOOP (:645-765) on the lane's texels
-S2TC_D uint64_t alpha_refine_lane(const uint32_t (&tx)[16], int w, int h, int refine, int &a0, int &a1)
+template <bool FULL>
+S2TC_D uint64_t alpha_refine_lane(const uint32_t (&tx)[16], const Masks &mk, int refine, int &a0, int &a1)
 {
 	uint32_t score;
 	int n0, s0, n1, s1;
@@ -160,7 +179,7 @@ S2TC_D uint64_t alpha_refine_lane(const uint32_t (&tx)[16], int w, int h, int re
 			a0 = a1;
 			a1 = t;
 		}
-		return alpha_try_lane(tx, w, h, a0, a1, score, n0, s0, n1, s1);
+		return alpha_try_lane<FULL>(tx, mk, a0, a1, score, n0, s0, n1, s1);
 	}
 	uint64_t bits;
 	if (refine == REFINE_LOOP) {
@@ -168,7 +187,7 @@ S2TC_D uint64_t alpha_refine_lane(const uint32_t (&tx)[16], int w, int h, int re
 		uint32_t s = 0x7fffffffu;
 		int na0 = a0, na1 = a1;
 		for (int it = 0; it < S2TC_LOOP_CAP; ++it) {
-			uint64_t b2 = alpha_try_lane(tx, w, h, na0, na1, score, n0, s0, n1, s1);
+			uint64_t b2 = alpha_try_lane<FULL>(tx, mk, na0, na1, score, n0, s0, n1, s1);
 			if (!(score < s)) break;
 			bits = b2;
 			s = score;
@@ -177,7 +196,7 @@ S2TC_D uint64_t alpha_refine_lane(const uint32_t (&tx)[16], int w, int h, int re
 			if (!eval_alpha(n0, s0, n1, s1, na0, na1)) break;
 		}
 	} else {
-		bits = alpha_try_lane(tx, w, h, a0, a1, score, n0, s0, n1, s1);
+		bits = alpha_try_lane<FULL>(tx, mk, a0, a1, score, n0, s0, n1, s1);
 		eval_alpha(n0, s0, n1, s1, a0, a1);
 	}
 	alpha_fix_order(bits, a0, a1);
@@ -185,8 +204,8 @@ S2TC_D uint64_t alpha_refine_lane(const uint32_t (&tx)[16], int w, int h, int re
 }
 
 // colour refinement NEVER/ALWAYS/LOOP (:767-860) on the lane's texels
-template <int CD, bool TRANS>
-S2TC_D uint32_t color_refine_lane(const uint32_t (&tx)[16], const typename Dist<CD>::F (&f)[16], int w, int h,
+template <int CD, bool TRANS, bool FULL>
+S2TC_D uint32_t color_refine_lane(const uint32_t (&tx)[16], const typename Dist<CD>::F (&f)[16], const Masks &mk,
                                   int refine, uint32_t &c0, uint32_t &c1)
 {
 	typedef ColorRefineLane<CD, TRANS> R;
@@ -198,26 +217,20 @@ S2TC_D uint32_t color_refine_lane(const uint32_t (&tx)[16], const typename Dist<
 			c0 = c1;
 			c1 = t;
 		}
-		return R::try_pass(tx, f, w, h, c0, c1, score, n1, s1);
+		return R::template try_pass<FULL>(tx, f, mk, c0, c1, score, n1, s1);
 	}
 	// totals over the texels that enter the accumulator, fixed per block
-	int ntot = 0;
+	const uint32_t usem = (FULL ? 0xffffu : mk.valid) & ~(TRANS ? mk.trans : 0u);
+	const int ntot = __builtin_popcount(usem);
 	uint32_t stot = 0;
 #pragma unroll
-	for (int k = 0; k < 16; ++k) {
-		const int x = k >> 2, y = k & 3;
-		bool use = x < w && y < h && !(TRANS && texel_a(tx[k]) == 0);
-		uint32_t v = texel565(tx[k]);
-		uint32_t pk = (v >> 11) | (((v >> 5) & 63u) << 10) | ((v & 31u) << 21);
-		ntot += use ? 1 : 0;
-		stot += use ? pk : 0u;
-	}
+	for (int k = 0; k < 16; ++k) stot += ((usem >> k) & 1u) ? pack_sum(tx[k]) : 0u;
 	uint32_t bits;
 	if (refine == REFINE_LOOP) {
 		bits = 0;
 		uint32_t s = 0x7fffffffu, n0v = c0, n1v = c1;
 		for (int it = 0; it < S2TC_LOOP_CAP; ++it) {
-			uint32_t b2 = R::try_pass(tx, f, w, h, n0v, n1v, score, n1, s1);
+			uint32_t b2 = R::template try_pass<FULL>(tx, f, mk, n0v, n1v, score, n1, s1);
 			if (!(score < s)) break;
 			bits = b2;
 			s = score;
@@ -226,7 +239,7 @@ S2TC_D uint32_t color_refine_lane(const uint32_t (&tx)[16], const typename Dist<
 			if (!eval_color(ntot - n1, stot - s1, n1, s1, n0v, n1v)) break;
 		}
 	} else {
-		bits = R::try_pass(tx, f, w, h, c0, c1, score, n1, s1);
+		bits = R::template try_pass<FULL>(tx, f, mk, c0, c1, score, n1, s1);
 		eval_color(ntot - n1, stot - s1, n1, s1, c0, c1);
 	}
 	color_fix_order<TRANS>(bits, c0, c1);
@@ -234,12 +247,13 @@ S2TC_D uint32_t color_refine_lane(const uint32_t (&tx)[16], const typename Dist<
 }
 
 // The whole quick-mode block.  tx[k] holds the quantized texel at x = k >> 2,
-// y = k & 3 (only x < w, y < h are read).
-template <int CD, int DXT>
-S2TC_D BlockOut encode_block_fast(const uint32_t (&tx)[16], int w, int h, int refine)
+// y = k & 3 (only x < w, y < h are read).  FULL: the block is 4x4.
+template <int CD, int DXT, bool FULL>
+S2TC_D BlockOut encode_block_fast_t(const uint32_t (&tx)[16], int w, int h, int refine)
 {
 	typedef Dist<CD> D;
 	typedef typename D::F F;
+	const Masks mk = block_masks<DXT>(tx, FULL ? 4 : w, FULL ? 4 : h);
 	const F black = D::feat(0u);
 	F f[16];
 	uint32_t c0 = 0xffffu, c1 = 0u;  // dummy (31,63,31) and (0,0,0) (:886-891)
@@ -247,42 +261,39 @@ S2TC_D BlockOut encode_block_fast(const uint32_t (&tx)[16], int w, int h, int re
 	int a0 = texel_a(tx[0]), a1 = a0;  // (:894-898)
 #pragma unroll
 	for (int k = 0; k < 16; ++k) {
-		const int x = k >> 2, y = k & 3;
-		const bool valid = x < w && y < h;
+		const bool valid = FULL || ((mk.valid >> k) & 1u);
+		const bool use = valid && !(DXT == DXT1 && ((mk.trans >> k) & 1u));
 		const uint32_t v = texel565(tx[k]);
 		f[k] = D::feat(v);
-		const bool use = valid && !(DXT == DXT1 && texel_a(tx[k]) == 0);
-		int d = D::d(f[k], black);
-		if (use && d > dmax) {
-			dmax = d;
-			c1 = v;
-		}
-		if (use && d < dmin) {
-			dmin = d;
-			c0 = v;
-		}
+		const int d = D::d(f[k], black);
+		const bool up = use && d > dmax;
+		dmax = up ? d : dmax;
+		c1 = up ? v : c1;
+		const bool dn = use && d < dmin;
+		dmin = dn ? d : dmin;
+		c0 = dn ? v : c0;
 		if (DXT == DXT5) {
-			int a = texel_a(tx[k]);
-			if (valid && a != 255) {
-				a1 = a > a1 ? a : a1;
-				a0 = a < a0 ? a : a0;
-			}
+			const int a = texel_a(tx[k]);
+			const bool t = valid && a != 255;
+			a1 = (t && a > a1) ? a : a1;
+			a0 = (t && a < a0) ? a : a0;
 		}
 	}
 	c1 = fix_equal565(c0, c1);
 	BlockOut o;
 	if (DXT == DXT1) {
-		uint32_t bits = color_refine_lane<CD, true>(tx, f, w, h, refine, c0, c1);
+		const uint32_t bits = color_refine_lane<CD, true, FULL>(tx, f, mk, refine, c0, c1);
 		o.w[0] = c0 | (c1 << 16);
 		o.w[1] = bits;
 		o.w[2] = o.w[3] = 0;
 	} else if (DXT == DXT3) {
-		uint32_t bits = color_refine_lane<CD, false>(tx, f, w, h, refine, c0, c1);
+		const uint32_t bits = color_refine_lane<CD, false, FULL>(tx, f, mk, refine, c0, c1);
 		uint64_t ab = 0;
 #pragma unroll
 		for (int k = 0; k < 16; ++k) {
 			const int x = k >> 2, y = k & 3;
-			if (x < w && y < h) ab |= (uint64_t)texel_a(tx[k]) << (4 * (y * 4 + x));
+			const bool valid = FULL || ((mk.valid >> k) & 1u);
+			ab |= valid ? (uint64_t)texel_a(tx[k]) << (4 * (y * 4 + x)) : 0ull;
 		}
 		o.w[0] = (uint32_t)ab;
 		o.w[1] = (uint32_t)(ab >> 32);
@@ -290,15 +301,22 @@ S2TC_D BlockOut encode_block_fast(const uint32_t (&tx)[16], int w, int h, int re
 		o.w[3] = bits;
 	} else {
 		a1 = fix_equal_a(a0, a1);
-		uint32_t bits = color_refine_lane<CD, false>(tx, f, w, h, refine, c0, c1);
-		uint64_t ab = alpha_refine_lane(tx, w, h, refine, a0, a1);
-		uint64_t lo = (uint64_t)(a0 & 255) | ((uint64_t)(a1 & 255) << 8) | (ab << 16);
+		const uint32_t bits = color_refine_lane<CD, false, FULL>(tx, f, mk, refine, c0, c1);
+		const uint64_t ab = alpha_refine_lane<FULL>(tx, mk, refine, a0, a1);
+		const uint64_t lo = (uint64_t)(a0 & 255) | ((uint64_t)(a1 & 255) << 8) | (ab << 16);
 		o.w[0] = (uint32_t)lo;
 		o.w[1] = (uint32_t)(lo >> 32);
 		o.w[2] = c0 | (c1 << 16);
 		o.w[3] = bits;
 	}
 	return o;
+}
+
+template <int CD, int DXT>
+S2TC_D BlockOut encode_block_fast(const uint32_t (&tx)[16], int w, int h, int refine)
+{
+	if (w == 4 && h == 4) return encode_block_fast_t<CD, DXT, true>(tx, 4, 4, refine);
+	return encode_block_fast_t<CD, DXT, false>(tx, w, h, refine);
 }
 
 }  // namespace s2tc

@@ -4,6 +4,7 @@
 // exports s2tc_kernels_cd<N>() returning its kernel table.  Splitting by mode
 // keeps each translation unit small and lets the build compile them in
 // parallel.
+#include "encode_all16.h"
 #include "encode_normal.h"
 #include "kernel_table.h"
 
@@ -76,6 +77,22 @@ __global__ __launch_bounds__(256) void k_fast(const uint32_t *__restrict__ src, 
 	uint32_t tx[16];
 	load_block<DXT, RAW>(src, width, bx, by, w, h, vec != 0, tx);
 	const BlockOut o = encode_block_fast<CD, DXT>(tx, w, h, refine);
+	store_block<DXT>(dst, (size_t)b, o);
+}
+
+// MODE_NORMAL without random colours over one image: one lane per block.
+template <int CD, int DXT, bool RAW>
+__global__ __launch_bounds__(256) void k_all16(const uint32_t *__restrict__ src, int width, int height,
+                                               uint8_t *__restrict__ dst, int refine, int vec)
+{
+	const int bw = (width + 3) >> 2, bh = (height + 3) >> 2;
+	const int b = (int)(blockIdx.x * 256 + threadIdx.x);
+	if (b >= bw * bh) return;
+	const int by = b / bw, bx = b - by * bw;
+	const int w = min(4, width - 4 * bx), h = min(4, height - 4 * by);
+	uint32_t tx[16];
+	load_block<DXT, RAW>(src, width, bx, by, w, h, vec != 0, tx);
+	const BlockOut o = encode_block_all16<CD, DXT>(tx, w, h, refine);
 	store_block<DXT>(dst, (size_t)b, o);
 }
 
@@ -152,6 +169,29 @@ __global__ __launch_bounds__(256) void k_fast_batch(const JobDesc *__restrict__ 
 	}
 }
 
+template <int CD, int DXT>
+__global__ __launch_bounds__(256) void k_all16_batch(const JobDesc *__restrict__ jobs,
+                                                     const TileJobs *__restrict__ tile_jobs, long long nblocks,
+                                                     int ntiles)
+{
+	for (int tile = (int)blockIdx.x; tile < ntiles; tile += (int)gridDim.x) {
+		const long long gb = (long long)tile * 256 + threadIdx.x;
+		if (gb >= nblocks) continue;
+		const TileJobs tj = tile_jobs[tile];
+		const JobDesc &j = jobs[find_job(jobs, tj.lo, tj.hi, gb)];
+		const int b = (int)(gb - j.block0);
+		const int bw = (j.width + 3) >> 2;
+		const int by = b / bw, bx = b - by * bw;
+		const int w = min(4, j.width - 4 * bx), h = min(4, j.height - 4 * by);
+		const bool vec = (j.width & 3) == 0;
+		uint32_t tx[16];
+		if (j.raw) load_block<DXT, true>(j.src, j.width, bx, by, w, h, vec, tx);
+		else load_block<DXT, false>(j.src, j.width, bx, by, w, h, vec, tx);
+		const BlockOut o = encode_block_all16<CD, DXT>(tx, w, h, j.refine);
+		store_block<DXT>(j.dst, (size_t)b, o);
+	}
+}
+
 template <int CD>
 __global__ __launch_bounds__(64) void k_normal_batch(const JobDesc *__restrict__ jobs, const TileJobs *__restrict__ tile_jobs,
                                                      long long nblocks, int ntiles, int pbw_max)
@@ -215,6 +255,15 @@ extern "C" void S2TC_CAT(s2tc_kernels_cd, S2TC_CD)(KernelTable *kt)
 	for (int d = 0; d < 3; ++d) kt->fast_batch[d] = nullptr;
 #endif
 	kt->normal_batch = (const void *)k_normal_batch<S2TC_CD>;
+	kt->all16[DXT1][0] = (const void *)k_all16<S2TC_CD, DXT1, false>;
+	kt->all16[DXT3][0] = (const void *)k_all16<S2TC_CD, DXT3, false>;
+	kt->all16[DXT5][0] = (const void *)k_all16<S2TC_CD, DXT5, false>;
+	kt->all16[DXT1][1] = (const void *)k_all16<S2TC_CD, DXT1, true>;
+	kt->all16[DXT3][1] = (const void *)k_all16<S2TC_CD, DXT3, true>;
+	kt->all16[DXT5][1] = (const void *)k_all16<S2TC_CD, DXT5, true>;
+	kt->all16_batch[DXT1] = (const void *)k_all16_batch<S2TC_CD, DXT1>;
+	kt->all16_batch[DXT3] = (const void *)k_all16_batch<S2TC_CD, DXT3>;
+	kt->all16_batch[DXT5] = (const void *)k_all16_batch<S2TC_CD, DXT5>;
 	kt->normal[0] = (const void *)k_normal<S2TC_CD, false>;
 	kt->normal[1] = (const void *)k_normal<S2TC_CD, true>;
 	kt->normal_smem_per_cand = (int)(4 * (sizeof(uint32_t) + sizeof(Dist<S2TC_CD>::F) + S2TC_RS * sizeof(int)));

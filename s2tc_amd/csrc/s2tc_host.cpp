@@ -4,7 +4,9 @@
 // The dispatch rules mirror s2tc_encode_block_func (s2tc_algorithm.cpp
 // :1110-1194): out-of-range refine -> ALWAYS, dxt -> DXT5, distance -> WAVG;
 // quick mode (one lane per block) iff nrandom < 0 and the distance is not
-// NORMALMAP; everything else runs the 16-lanes-per-block MODE_NORMAL kernel.
+// NORMALMAP; MODE_NORMAL without random colours (nrandom <= 0) runs the
+// one-lane-per-block all-16 kernel, MODE_NORMAL with random colours the
+// 16-lanes-per-block kernel.
 #include "s2tc_internal.h"
 
 #include <cstdarg>
@@ -95,6 +97,12 @@ int launch_encode(const EncodeArgs &a, hipStream_t stream)
 		void *args[] = {(void *)&src, (void *)&width, (void *)&height, (void *)&dst, (void *)&refine, (void *)&vec};
 		dim3 grid((unsigned)((nb + 255) / 256)), block(256);
 		return hip_check(hipLaunchKernel(kt.fast[dxt][raw], grid, block, args, 0, stream), "launch k_fast");
+	}
+	if (a.nrandom <= 0) {  // only the block's colours: one lane per block
+		int vec = ((width & 3) == 0 && (((uintptr_t)src) & 15) == 0) ? 1 : 0;
+		void *args[] = {(void *)&src, (void *)&width, (void *)&height, (void *)&dst, (void *)&refine, (void *)&vec};
+		dim3 grid((unsigned)((nb + 255) / 256)), block(256);
+		return hip_check(hipLaunchKernel(kt.all16[dxt][raw], grid, block, args, 0, stream), "launch k_all16");
 	}
 	int nrandom = a.nrandom;
 	int pbw = normal_pbw(nrandom);
