@@ -99,6 +99,44 @@ def cpu_baseline(items, budget_s=12.0):
                        f"DITHER_NONE quantization + the reference's block loop, {n} threads, {t_used:.1f} s")
 
 
+def serve_leg(s, rates, duration, npool=256, seed=4):
+    """Open-loop Poisson arrivals of textures (with mip chains) drawn from a
+    pool of the serving mix (70% quick mode, 30% nrandom 64 + LOOP), sources
+    resident on the device, 64 GiB admission budget; continuous batching on 4
+    streams vs one-at-a-time plain launches."""
+    import numpy as np
+    from s2tc_amd import workloads
+    jobs = workloads.batch_jobs(seed, npool, serve_mix=True)
+    items, groups = [], []
+    for t in jobs.textures:
+        f = len(items)
+        for lv in t["levels"]:
+            items.append(dict(rgba=lv["rgba"], width=lv["width"], height=lv["height"], dxt=t["dxt"], cd=t["cd"],
+                              nrandom=t["nrandom"], refine=t["refine"], seed=lv["seed"]))
+        groups.append((f, len(items) - f))
+    imgs = s.DeviceImages(items)
+    out = {}
+    for mode in ("batched", "serial_plain"):
+        for rate in rates:
+            if mode == "serial_plain" and rate > 10000:
+                continue
+            rng = np.random.default_rng(rate)
+            n = max(1, int(rate * duration))
+            arr = np.cumsum(rng.exponential(1.0 / rate, n))
+            pick = rng.integers(0, npool, n)
+            reqs = [(float(a), groups[p][0], groups[p][1]) for a, p in zip(arr, pick)]
+            st, lat = s.serve(imgs, reqs, hbm_budget=64 << 30, streams=4, max_batch=256,
+                              serial_plain=(mode == "serial_plain"))
+            out[f"{mode}@{rate}"] = dict(requests=n, p50_ms=round(st["p50_ms"], 3), p99_ms=round(st["p99_ms"], 3),
+                                        mean_ms=round(st["mean_ms"], 3), max_ms=round(st["max_ms"], 3),
+                                        mtexels_s=round(st["texels"] / max(st["wall_s"], 1e-9) / 1e6, 1),
+                                        batches=st["batches"], max_queue=st["max_queue"],
+                                        offered_mtexels_s=round(st["texels"] / arr[-1] / 1e6, 1))
+            log(f"serve {mode}@{rate}: {out[f'{mode}@{rate}']}")
+    imgs.free()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -108,12 +146,14 @@ def main():
     ap.add_argument("--schedule", default="", help="wg_per_cu,streams (skips the autotuner)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-duration", type=float, default=0.5)
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="only run a plain device copy of this many MiB (PMC calibration) and exit")
     a = ap.parse_args()
     if a.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (asked for {a.gpus}); multi-GPU sharding is "
-              "described in DESIGN.md, not run.", file=sys.stderr)
+        print(f"bench.py: measures one GPU only (asked for {a.gpus}); textures shard across GPUs by "
+              "whole job with no communication, so N GPUs are N independent copies of this run.", file=sys.stderr)
         sys.exit(2)
     import numpy as np
     import s2tc_amd as s
@@ -189,6 +229,10 @@ def main():
         plain = dict(ms_per_step=round(pdt * 1e3, 3), value=round(texels / pdt / 1e6, 1),
                      launches=len(items), speedup_batched=round(pdt / dt, 2))
 
+    serve = None
+    if not a.no_serve and a.workload == "batch":
+        serve = serve_leg(s, (2000, 10000, 50000), a.serve_duration)
+
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(items)
@@ -208,6 +252,7 @@ def main():
         "roofline": roofline,
         "classes": class_summary,
         "cpu_baseline": cpu,
+        "serve": serve,
     }
     print(json.dumps(out), flush=True)
     batch.destroy()

@@ -118,6 +118,41 @@ S2TC_API int s2tc_amd_batch_destroy(s2tc_amd_batch *b);
 /* The same jobs with one plain kernel launch per job (the per-texture path). */
 S2TC_API int s2tc_amd_encode_jobs_plain(const s2tc_amd_job *jobs, int njobs, void *stream);
 
+/* ---- serving: requests arriving over time, continuous batching ---- */
+
+typedef struct {
+	double arrival_s;        /* arrival time in seconds after the start of the run (sorted) */
+	int first_job, njobs;    /* the request's images: jobs[first_job .. first_job + njobs) */
+} s2tc_amd_request;
+
+typedef struct {
+	size_t hbm_budget;       /* admitted, unfinished requests hold source + output bytes against it */
+	int streams;             /* batches in flight at once (1..16), each on its own stream */
+	int max_batch;           /* most requests in one dispatched batch */
+	int serial_plain;        /* 1: one request at a time with one plain launch per image */
+	int wg_per_cu;           /* persistent grid of the class kernels (0: one workgroup per tile) */
+	void **keep_outputs;     /* NULL, or one host buffer per job receiving its blocks */
+} s2tc_amd_serve_opts;
+
+typedef struct {
+	int completed, rejected; /* rejected: a request larger than the budget */
+	int batches;
+	int max_queue;           /* most requests waiting for admission at once */
+	int max_inflight;        /* most requests admitted and unfinished at once */
+	double wall_s;           /* start to last completion */
+	long long texels;        /* texels of completed requests */
+	double p50_ms, p99_ms, mean_ms, max_ms;  /* arrival-to-completion latency */
+} s2tc_amd_serve_stats;
+
+/* Runs the requests in real time: each arrives at its arrival_s, waits for
+ * admission under the HBM budget, is batched with the other admitted
+ * requests onto a free stream and completes when its batch has finished.
+ * Job dst pointers are ignored: outputs live in a device arena owned by the
+ * run (copied to keep_outputs when given).  latency_ms (nreq entries, may be
+ * NULL) gets each request's latency, -1 for rejected ones. */
+S2TC_API int s2tc_amd_serve(const s2tc_amd_job *jobs, int njobs, const s2tc_amd_request *reqs, int nreq,
+                            const s2tc_amd_serve_opts *opts, double *latency_ms, s2tc_amd_serve_stats *stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -258,3 +258,44 @@ class Batch:
 def encode_jobs_plain(images, stream=None):
     L = _bind_batch(lib())
     check(L.s2tc_amd_encode_jobs_plain(ctypes.cast(images.jobs, _P), len(images.jobs), stream))
+
+
+# ---- serving ----
+
+class Request(ctypes.Structure):
+    """s2tc_amd_request"""
+    _fields_ = [("arrival_s", ctypes.c_double), ("first_job", _I), ("njobs", _I)]
+
+
+class ServeOpts(ctypes.Structure):
+    """s2tc_amd_serve_opts"""
+    _fields_ = [("hbm_budget", _SZ), ("streams", _I), ("max_batch", _I), ("serial_plain", _I),
+                ("wg_per_cu", _I), ("keep_outputs", ctypes.POINTER(_P))]
+
+
+class ServeStats(ctypes.Structure):
+    """s2tc_amd_serve_stats"""
+    _fields_ = [("completed", _I), ("rejected", _I), ("batches", _I), ("max_queue", _I), ("max_inflight", _I),
+                ("wall_s", ctypes.c_double), ("texels", ctypes.c_longlong), ("p50_ms", ctypes.c_double),
+                ("p99_ms", ctypes.c_double), ("mean_ms", ctypes.c_double), ("max_ms", ctypes.c_double)]
+
+
+def serve(images, requests, hbm_budget=64 << 30, streams=4, max_batch=256, serial_plain=False, wg_per_cu=8,
+          keep_outputs=None):
+    """Run requests [(arrival_s, first_job, njobs)] over the images of a
+    DeviceImages in real time.  Returns (stats dict, latencies_ms array).
+    keep_outputs: optional list (one per image) of uint8 arrays to receive blocks."""
+    L = lib()
+    L.s2tc_amd_serve.argtypes = [_P, _I, _P, _I, ctypes.POINTER(ServeOpts), _P, ctypes.POINTER(ServeStats)]
+    reqs = (Request * len(requests))(*[Request(a, f, n) for (a, f, n) in requests])
+    keep = None
+    if keep_outputs is not None:
+        keep = (_P * len(images.jobs))(*[o.ctypes.data if o is not None else None for o in keep_outputs])
+    opts = ServeOpts(int(hbm_budget), streams, max_batch, 1 if serial_plain else 0, wg_per_cu,
+                     ctypes.cast(keep, ctypes.POINTER(_P)) if keep is not None else None)
+    lat = np.zeros(len(requests), np.float64)
+    st = ServeStats()
+    check(L.s2tc_amd_serve(ctypes.cast(images.jobs, _P), len(images.jobs), ctypes.cast(reqs, _P), len(requests),
+                           ctypes.byref(opts), lat.ctypes.data_as(_P), ctypes.byref(st)))
+    stats = {k: getattr(st, k) for k, _ in ServeStats._fields_}
+    return stats, lat

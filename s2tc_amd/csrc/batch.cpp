@@ -13,7 +13,7 @@
 // captured once into a hipGraph, spread over `streams` capture streams so
 // that small classes overlap with large ones; a launch of the batch is one
 // hipGraphLaunch.
-#include "s2tc_internal.h"
+#include "plan.h"
 
 #include <algorithm>
 #include <cstdio>
@@ -21,22 +21,6 @@
 #include <vector>
 
 namespace s2tc {
-
-struct ClassPlan {
-	bool normal = false;         // 16-lanes-per-block kernel
-	bool all16 = false;          // one-lane-per-block MODE_NORMAL kernel
-	int cd = 0, dxt = -1;
-	std::vector<JobDesc> jobs;
-	std::vector<int> job_index;  // index into the caller's job array
-	long long nblocks = 0, texels = 0, bytes = 0;
-	int ntiles = 0, tile = 256, pbw_max = 1;
-	size_t smem = 0;
-	const void *kernel = nullptr;
-	int occupancy = 1;           // resident workgroups per CU
-	JobDesc *d_jobs = nullptr;
-	TileJobs *d_tiles = nullptr;
-	double cost = 0;             // relative cost estimate, for ordering
-};
 
 }  // namespace s2tc
 
@@ -48,27 +32,10 @@ struct s2tc_amd_batch {
 	hipGraphExec_t exec = nullptr;
 	std::vector<hipStream_t> cap_streams;
 	std::vector<hipEvent_t> cap_events;
+	void *d_desc = nullptr;  // device copy of every class's job and tile descriptors
 };
 
 namespace s2tc {
-
-static int launch_class(const ClassPlan &c, int wg_per_cu, int ncu, hipStream_t st)
-{
-	long long nb = c.nblocks;
-	int ntiles = c.ntiles;
-	int grid = wg_per_cu <= 0 ? ntiles : std::min(ntiles, wg_per_cu * ncu);
-	if (grid < 1) return 0;
-	const JobDesc *jobs = c.d_jobs;
-	const TileJobs *tiles = c.d_tiles;
-	if (c.normal) {
-		int pbw = c.pbw_max;
-		void *args[] = {(void *)&jobs, (void *)&tiles, (void *)&nb, (void *)&ntiles, (void *)&pbw};
-		return hip_check(hipLaunchKernel(c.kernel, dim3(grid), dim3(64), args, c.smem, st), "launch k_normal_batch");
-	}
-	void *args[] = {(void *)&jobs, (void *)&tiles, (void *)&nb, (void *)&ntiles};
-	return hip_check(hipLaunchKernel(c.kernel, dim3(grid), dim3(256), args, 0, st),
-	                 c.all16 ? "launch k_all16_batch" : "launch k_fast_batch");
-}
 
 static void destroy_graph(s2tc_amd_batch *b)
 {
@@ -92,7 +59,7 @@ static int capture(s2tc_amd_batch *b)
 	for (int i = 1; i < ns && !rc; ++i)
 		rc = hip_check(hipStreamWaitEvent(b->cap_streams[i], b->cap_events[0], 0), "hipStreamWaitEvent");
 	for (size_t k = 0; k < b->classes.size() && !rc; ++k)
-		rc = launch_class(b->classes[k], b->wg_per_cu, b->ncu, b->cap_streams[k % ns]);
+		rc = plan_launch_class(b->classes[k], b->wg_per_cu, b->ncu, b->cap_streams[k % ns]);
 	for (int i = 1; i < ns && !rc; ++i) {
 		rc = hip_check(hipEventRecord(b->cap_events[i], b->cap_streams[i]), "hipEventRecord");
 		if (!rc) rc = hip_check(hipStreamWaitEvent(s0, b->cap_events[i], 0), "hipStreamWaitEvent");
@@ -111,11 +78,8 @@ static int capture(s2tc_amd_batch *b)
 static void free_plan(s2tc_amd_batch *b)
 {
 	destroy_graph(b);
-	for (auto &c : b->classes) {
-		if (c.d_jobs) (void)hipFree(c.d_jobs);
-		c.d_jobs = nullptr;
-		c.d_tiles = nullptr;
-	}
+	if (b->d_desc) (void)hipFree(b->d_desc);
+	b->d_desc = nullptr;
 	for (auto s : b->cap_streams) (void)hipStreamDestroy(s);
 	for (auto e : b->cap_events) (void)hipEventDestroy(e);
 	b->cap_streams.clear();
@@ -134,102 +98,16 @@ int s2tc_amd_batch_create(const s2tc_amd_job *jobs, int njobs, s2tc_amd_batch **
 	*out = nullptr;
 	if (int r = ensure_device()) return r;
 	s2tc_amd_batch *b = new s2tc_amd_batch;
-	int dev = 0;
-	(void)hipGetDevice(&dev);
-	hipDeviceProp_t prop;
-	if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
-		b->ncu = prop.multiProcessorCount;
-	// classes: 0..23 quick (cd*3 + dxt), 24..31 MODE_NORMAL with random
-	// colours (cd), 32..55 MODE_NORMAL without (32 + cd*3 + dxt)
-	std::vector<int> slot(56, -1);
-	for (int i = 0; i < njobs; ++i) {
-		const s2tc_amd_job &j = jobs[i];
-		int dxt = j.dxt, cd = j.cd, refine = j.refine;
-		normalize_modes(dxt, cd, refine);
-		if (j.width <= 0 || j.height <= 0 || !j.src || !j.dst) {
-			delete b;
-			return set_error(S2TC_E_ARG, "job %d: bad image", i);
+	b->ncu = device_cu_count();
+	int rc = plan_classes(jobs, njobs, b->classes);
+	if (!rc) {
+		const size_t bytes = plan_descriptor_bytes(b->classes);
+		std::vector<uint8_t> host(bytes);
+		rc = hip_check(hipMalloc(&b->d_desc, bytes ? bytes : 16), "hipMalloc (batch plan)");
+		if (!rc) {
+			plan_pack(b->classes, host.data(), (uint8_t *)b->d_desc);
+			rc = hip_check(hipMemcpy(b->d_desc, host.data(), bytes, hipMemcpyHostToDevice), "hipMemcpy (plan)");
 		}
-		if (j.nrandom > S2TC_MAX_NRANDOM) {
-			delete b;
-			return set_error(S2TC_E_RANGE, "job %d: nrandom %d exceeds the GPU limit %d", i, j.nrandom,
-			                 S2TC_MAX_NRANDOM);
-		}
-		if (j.src_kind != S2TC_SRC_QUANTIZED && j.src_kind != S2TC_SRC_RGBA8) {
-			delete b;
-			return set_error(S2TC_E_ARG, "job %d: bad source kind", i);
-		}
-		const bool fast = is_fast(cd, j.nrandom);
-		const bool all16 = !fast && j.nrandom <= 0;
-		const int key = fast ? cd * 3 + dxt : all16 ? 32 + cd * 3 + dxt : 24 + cd;
-		if (slot[key] < 0) {
-			slot[key] = (int)b->classes.size();
-			b->classes.emplace_back();
-			ClassPlan &c = b->classes.back();
-			c.normal = !fast && !all16;
-			c.all16 = all16;
-			c.cd = cd;
-			c.dxt = c.normal ? -1 : dxt;
-			c.tile = c.normal ? 4 : 256;
-		}
-		ClassPlan &c = b->classes[slot[key]];
-		JobDesc d;
-		d.src = (const uint32_t *)j.src;
-		d.dst = (uint8_t *)j.dst;
-		d.width = j.width;
-		d.height = j.height;
-		d.dxt = dxt;
-		d.nrandom = fast ? -1 : (j.nrandom > 0 ? j.nrandom : 0);
-		d.refine = refine;
-		d.raw = j.src_kind == S2TC_SRC_RGBA8 ? 1 : 0;
-		d.seed = j.seed;
-		d.block0 = c.nblocks;
-		const long long nb = (long long)((j.width + 3) / 4) * ((j.height + 3) / 4);
-		c.nblocks += nb;
-		c.texels += (long long)j.width * j.height;
-		c.bytes += (long long)j.width * j.height * 4 + nb * block_bytes(dxt);
-		c.pbw_max = std::max(c.pbw_max, normal_pbw(d.nrandom));
-		c.jobs.push_back(d);
-		c.job_index.push_back(i);
-		// rough cost: quick mode ~1 per block, MODE_NORMAL ~ candidate pairs
-		const int m = 16 + std::max(0, d.nrandom);
-		c.cost += fast ? (double)nb : all16 ? (double)nb * 8.0 * (dxt == DXT5 ? 2 : 1)
-		                                    : (double)nb * (m * (m - 1) / 2) * (dxt == DXT5 ? 2 : 1) / 8.0;
-	}
-	int rc = 0;
-	for (auto &c : b->classes) {
-		const KernelTable &kt = kernel_table(c.cd);
-		c.kernel = c.normal ? kt.normal_batch : c.all16 ? kt.all16_batch[c.dxt] : kt.fast_batch[c.dxt];
-		if (!c.kernel) {
-			rc = set_error(S2TC_E_ARG, "no batch kernel for distance %d", c.cd);
-			break;
-		}
-		c.ntiles = (int)((c.nblocks + c.tile - 1) / c.tile);
-		c.smem = c.normal ? (size_t)kt.normal_smem_per_cand * 16 * c.pbw_max : 0;
-		// tile -> (first job, last job), by a walk over the packed blocks
-		std::vector<TileJobs> tiles((size_t)c.ntiles);
-		int jlo = 0;
-		const int nj = (int)c.jobs.size();
-		for (int t = 0; t < c.ntiles; ++t) {
-			const long long b0 = (long long)t * c.tile, b1 = std::min(c.nblocks, b0 + c.tile) - 1;
-			while (jlo + 1 < nj && c.jobs[jlo + 1].block0 <= b0) ++jlo;
-			int jhi = jlo;
-			while (jhi + 1 < nj && c.jobs[jhi + 1].block0 <= b1) ++jhi;
-			tiles[(size_t)t] = TileJobs{jlo, jhi};
-		}
-		const size_t jb = c.jobs.size() * sizeof(JobDesc), tb = tiles.size() * sizeof(TileJobs);
-		void *mem = nullptr;
-		rc = hip_check(hipMalloc(&mem, jb + tb + 16), "hipMalloc (batch plan)");
-		if (rc) break;
-		c.d_jobs = (JobDesc *)mem;
-		c.d_tiles = (TileJobs *)((uint8_t *)mem + ((jb + 15) & ~(size_t)15));
-		rc = hip_check(hipMemcpy(c.d_jobs, c.jobs.data(), jb, hipMemcpyHostToDevice), "hipMemcpy (jobs)");
-		if (!rc) rc = hip_check(hipMemcpy(c.d_tiles, tiles.data(), tb, hipMemcpyHostToDevice), "hipMemcpy (tiles)");
-		if (rc) break;
-		int occ = 1;
-		if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, c.kernel, c.normal ? 64 : 256, c.smem) == hipSuccess &&
-		    occ > 0)
-			c.occupancy = occ;
 	}
 	if (!rc) {
 		std::stable_sort(b->classes.begin(), b->classes.end(),
@@ -336,9 +214,9 @@ int s2tc_amd_batch_class_times(s2tc_amd_batch *b, void *stream, s2tc_amd_class_t
 	}
 	int rc = 0, n = 0;
 	for (const ClassPlan &c : b->classes) {
-		rc = launch_class(c, b->wg_per_cu, b->ncu, st);  // warm-up
+		rc = plan_launch_class(c, b->wg_per_cu, b->ncu, st);  // warm-up
 		if (!rc) rc = hip_check(hipEventRecord(e0, st), "hipEventRecord");
-		if (!rc) rc = launch_class(c, b->wg_per_cu, b->ncu, st);
+		if (!rc) rc = plan_launch_class(c, b->wg_per_cu, b->ncu, st);
 		if (!rc) rc = hip_check(hipEventRecord(e1, st), "hipEventRecord");
 		if (!rc) rc = hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
 		if (rc) break;

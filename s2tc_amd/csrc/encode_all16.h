@@ -121,7 +121,7 @@ S2TC_D BlockOut encode_block_all16(const uint32_t (&tx)[16], int w, int h, int r
 			{
 				F f[16];
 #pragma unroll
-				for (int k = 0; k < 16; ++k) f[k] = Dd::feat(texel565(tx[k]));
+				for (int k = 0; k < 16; ++k) f[k] = Dd::featq(tx[k]);
 #pragma unroll
 				for (int i = 0; i < 16; ++i)
 #pragma unroll
@@ -163,9 +163,8 @@ S2TC_D BlockOut encode_block_all16(const uint32_t (&tx)[16], int w, int h, int r
 			}
 		}
 	}
-	F f[16];
-#pragma unroll
-	for (int k = 0; k < 16; ++k) f[k] = Dd::feat(texel565(tx[k]));
+	TexFeat<CD> f;
+	f.init(tx);
 	const Masks mk = block_masks<DXT>(tx, w, h);
 	c1 = fix_equal565(c0, c1);
 	BlockOut o;

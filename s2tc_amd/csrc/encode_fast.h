@@ -53,7 +53,7 @@ struct ColorRefineLane {
 	// without branches: texels outside the block get index 0 and do not
 	// count, transparent DXT1 texels get index 3 and do not count.
 	template <bool FULL>
-	static S2TC_D uint32_t try_pass(const uint32_t (&tx)[16], const F (&f)[16], const Masks &mk, uint32_t c0,
+	static S2TC_D uint32_t try_pass(const uint32_t (&tx)[16], const TexFeat<CD> &f, const Masks &mk, uint32_t c0,
 	                                uint32_t c1, uint32_t &score, int &n1, uint32_t &s1)
 	{
 		const F e0 = D::feat(c0), e1 = D::feat(c1);
@@ -67,7 +67,8 @@ struct ColorRefineLane {
 			const bool valid = FULL || ((mk.valid >> k) & 1u);
 			const bool transp = TRANS && ((mk.trans >> k) & 1u);
 			const bool use = valid && !transp;
-			const int d0 = D::d(f[k], e0), d1 = D::d(f[k], e1);
+			const F fk = f.get(tx, k);
+			const int d0 = D::d(fk, e0), d1 = D::d(fk, e1);
 			const bool b = d1 < d0;
 			score += use ? (uint32_t)(b ? d1 : d0) : 0u;
 			const bool b1 = use && b;
@@ -205,8 +206,8 @@ S2TC_D uint64_t alpha_refine_lane(const uint32_t (&tx)[16], const Masks &mk, int
 
 // colour refinement NEVER/ALWAYS/LOOP (:767-860) on the lane's texels
 template <int CD, bool TRANS, bool FULL>
-S2TC_D uint32_t color_refine_lane(const uint32_t (&tx)[16], const typename Dist<CD>::F (&f)[16], const Masks &mk,
-                                  int refine, uint32_t &c0, uint32_t &c1)
+S2TC_D uint32_t color_refine_lane(const uint32_t (&tx)[16], const TexFeat<CD> &f, const Masks &mk, int refine,
+                                  uint32_t &c0, uint32_t &c1)
 {
 	typedef ColorRefineLane<CD, TRANS> R;
 	uint32_t score, s1;
@@ -255,7 +256,8 @@ S2TC_D BlockOut encode_block_fast_t(const uint32_t (&tx)[16], int w, int h, int 
 	typedef typename D::F F;
 	const Masks mk = block_masks<DXT>(tx, FULL ? 4 : w, FULL ? 4 : h);
 	const F black = D::feat(0u);
-	F f[16];
+	TexFeat<CD> f;
+	f.init(tx);
 	uint32_t c0 = 0xffffu, c1 = 0u;  // dummy (31,63,31) and (0,0,0) (:886-891)
 	int dmin = 0x7fffffff, dmax = 0;
 	int a0 = texel_a(tx[0]), a1 = a0;  // (:894-898)
@@ -264,8 +266,7 @@ S2TC_D BlockOut encode_block_fast_t(const uint32_t (&tx)[16], int w, int h, int 
 		const bool valid = FULL || ((mk.valid >> k) & 1u);
 		const bool use = valid && !(DXT == DXT1 && ((mk.trans >> k) & 1u));
 		const uint32_t v = texel565(tx[k]);
-		f[k] = D::feat(v);
-		const int d = D::d(f[k], black);
+		const int d = D::d(f.get(tx, k), black);
 		const bool up = use && d > dmax;
 		dmax = up ? d : dmax;
 		c1 = up ? v : c1;

@@ -1,0 +1,143 @@
+// plan.cpp - grouping of images into kernel classes (see plan.h).
+#include "plan.h"
+
+#include <algorithm>
+#include <cstring>
+
+namespace s2tc {
+
+int device_cu_count()
+{
+	static int ncu = 0;
+	if (!ncu) {
+		int dev = 0;
+		(void)hipGetDevice(&dev);
+		hipDeviceProp_t prop;
+		ncu = (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
+		          ? prop.multiProcessorCount
+		          : 256;
+	}
+	return ncu;
+}
+
+int plan_classes(const s2tc_amd_job *jobs, int njobs, std::vector<ClassPlan> &classes)
+{
+	classes.clear();
+	// class slots: 0..23 quick (cd*3 + dxt), 24..31 MODE_NORMAL with random
+	// colours (cd), 32..55 MODE_NORMAL without (32 + cd*3 + dxt)
+	int slot[56];
+	for (int &v : slot) v = -1;
+	for (int i = 0; i < njobs; ++i) {
+		const s2tc_amd_job &j = jobs[i];
+		int dxt = j.dxt, cd = j.cd, refine = j.refine;
+		normalize_modes(dxt, cd, refine);
+		if (j.width <= 0 || j.height <= 0 || !j.src || !j.dst) return set_error(S2TC_E_ARG, "job %d: bad image", i);
+		if (j.nrandom > S2TC_MAX_NRANDOM)
+			return set_error(S2TC_E_RANGE, "job %d: nrandom %d exceeds the GPU limit %d", i, j.nrandom,
+			                 S2TC_MAX_NRANDOM);
+		if (j.src_kind != S2TC_SRC_QUANTIZED && j.src_kind != S2TC_SRC_RGBA8)
+			return set_error(S2TC_E_ARG, "job %d: bad source kind", i);
+		const bool fast = is_fast(cd, j.nrandom);
+		const bool all16 = !fast && j.nrandom <= 0;
+		const int key = fast ? cd * 3 + dxt : all16 ? 32 + cd * 3 + dxt : 24 + cd;
+		if (slot[key] < 0) {
+			slot[key] = (int)classes.size();
+			classes.emplace_back();
+			ClassPlan &c = classes.back();
+			c.normal = !fast && !all16;
+			c.all16 = all16;
+			c.cd = cd;
+			c.dxt = c.normal ? -1 : dxt;
+			c.tile = c.normal ? 4 : 256;
+		}
+		ClassPlan &c = classes[slot[key]];
+		JobDesc d;
+		d.src = (const uint32_t *)j.src;
+		d.dst = (uint8_t *)j.dst;
+		d.width = j.width;
+		d.height = j.height;
+		d.dxt = dxt;
+		d.nrandom = fast ? -1 : (j.nrandom > 0 ? j.nrandom : 0);
+		d.refine = refine;
+		d.raw = j.src_kind == S2TC_SRC_RGBA8 ? 1 : 0;
+		d.seed = j.seed;
+		d.block0 = c.nblocks;
+		const long long nb = (long long)((j.width + 3) / 4) * ((j.height + 3) / 4);
+		c.nblocks += nb;
+		c.texels += (long long)j.width * j.height;
+		c.bytes += (long long)j.width * j.height * 4 + nb * block_bytes(dxt);
+		c.pbw_max = std::max(c.pbw_max, normal_pbw(d.nrandom));
+		c.jobs.push_back(d);
+		c.job_index.push_back(i);
+		// rough cost: quick mode ~1 per block, MODE_NORMAL ~ candidate pairs
+		const int m = 16 + std::max(0, d.nrandom);
+		c.cost += fast ? (double)nb : all16 ? (double)nb * 8.0 * (dxt == DXT5 ? 2 : 1)
+		                                    : (double)nb * (m * (m - 1) / 2) * (dxt == DXT5 ? 2 : 1) / 8.0;
+	}
+	for (auto &c : classes) {
+		const KernelTable &kt = kernel_table(c.cd);
+		c.kernel = c.normal ? kt.normal_batch : c.all16 ? kt.all16_batch[c.dxt] : kt.fast_batch[c.dxt];
+		if (!c.kernel) return set_error(S2TC_E_ARG, "no batch kernel for distance %d", c.cd);
+		c.ntiles = (int)((c.nblocks + c.tile - 1) / c.tile);
+		c.smem = c.normal ? (size_t)kt.normal_smem_per_cand * 16 * c.pbw_max : 0;
+		// tile -> (first job, last job), by a walk over the packed blocks
+		c.tiles.resize((size_t)c.ntiles);
+		int jlo = 0;
+		const int nj = (int)c.jobs.size();
+		for (int t = 0; t < c.ntiles; ++t) {
+			const long long b0 = (long long)t * c.tile, b1 = std::min(c.nblocks, b0 + c.tile) - 1;
+			while (jlo + 1 < nj && c.jobs[jlo + 1].block0 <= b0) ++jlo;
+			int jhi = jlo;
+			while (jhi + 1 < nj && c.jobs[jhi + 1].block0 <= b1) ++jhi;
+			c.tiles[(size_t)t] = TileJobs{jlo, jhi};
+		}
+	}
+	std::stable_sort(classes.begin(), classes.end(),
+	                 [](const ClassPlan &x, const ClassPlan &y) { return x.cost > y.cost; });
+	return 0;
+}
+
+static size_t align16(size_t n) { return (n + 15) & ~(size_t)15; }
+
+size_t plan_descriptor_bytes(const std::vector<ClassPlan> &classes)
+{
+	size_t n = 0;
+	for (const auto &c : classes)
+		n += align16(c.jobs.size() * sizeof(JobDesc)) + align16(c.tiles.size() * sizeof(TileJobs));
+	return n;
+}
+
+void plan_pack(std::vector<ClassPlan> &classes, uint8_t *host, uint8_t *dev)
+{
+	size_t off = 0;
+	for (auto &c : classes) {
+		const size_t jb = c.jobs.size() * sizeof(JobDesc), tb = c.tiles.size() * sizeof(TileJobs);
+		memcpy(host + off, c.jobs.data(), jb);
+		c.d_jobs = (JobDesc *)(dev + off);
+		off += align16(jb);
+		memcpy(host + off, c.tiles.data(), tb);
+		c.d_tiles = (TileJobs *)(dev + off);
+		off += align16(tb);
+	}
+}
+
+int plan_launch_class(const ClassPlan &c, int wg_per_cu, int ncu, hipStream_t st)
+{
+	long long nb = c.nblocks;
+	int ntiles = c.ntiles;
+	int grid = wg_per_cu <= 0 ? ntiles : std::min(ntiles, wg_per_cu * ncu);
+	if (grid < 1) return 0;
+	const JobDesc *jobs = c.d_jobs;
+	const TileJobs *tiles = c.d_tiles;
+	if (c.normal) {
+		int pbw = c.pbw_max;
+		void *args[] = {(void *)&jobs, (void *)&tiles, (void *)&nb, (void *)&ntiles, (void *)&pbw};
+		return hip_check(hipLaunchKernel(c.kernel, dim3(grid), dim3(64), args, c.smem, st), "launch k_normal_batch");
+	}
+	void *args[] = {(void *)&jobs, (void *)&tiles, (void *)&nb, (void *)&ntiles};
+	return hip_check(hipLaunchKernel(c.kernel, dim3(grid), dim3(256), args, 0, st),
+	                 c.all16 ? "launch k_all16_batch" : "launch k_fast_batch");
+}
+
+
+}  // namespace s2tc

@@ -1,0 +1,46 @@
+// plan.h - grouping of images into kernel classes for the batch and serving paths.
+#pragma once
+
+#include <vector>
+
+#include "s2tc_internal.h"
+
+namespace s2tc {
+
+// One kernel class of a batch: the jobs that run the same kernel.
+//   quick mode (nrandom < 0, distance != NORMALMAP): one class per
+//     (distance, format), k_fast_batch, one lane per block, tiles of 256;
+//   MODE_NORMAL without random colours: one class per (distance, format),
+//     k_all16_batch, one lane per block, tiles of 256;
+//   MODE_NORMAL with random colours: one class per distance (formats mixed),
+//     k_normal_batch, 16 lanes per block, tiles of 4 blocks.
+struct ClassPlan {
+	bool normal = false;         // 16-lanes-per-block kernel
+	bool all16 = false;          // one-lane-per-block MODE_NORMAL kernel
+	int cd = 0, dxt = -1;
+	std::vector<JobDesc> jobs;
+	std::vector<TileJobs> tiles;
+	std::vector<int> job_index;  // index into the caller's job array
+	long long nblocks = 0, texels = 0, bytes = 0;
+	int ntiles = 0, tile = 256, pbw_max = 1;
+	size_t smem = 0;
+	const void *kernel = nullptr;
+	JobDesc *d_jobs = nullptr;   // device addresses (set by plan_pack)
+	TileJobs *d_tiles = nullptr;
+	double cost = 0;             // relative cost estimate, for ordering
+};
+
+int device_cu_count();
+// Groups jobs into classes (sorted by decreasing cost) with their job and
+// tile descriptors on the host; validates every job.
+int plan_classes(const s2tc_amd_job *jobs, int njobs, std::vector<ClassPlan> &classes);
+// Bytes of device memory that hold all descriptors of the classes.
+size_t plan_descriptor_bytes(const std::vector<ClassPlan> &classes);
+// Lays the descriptors out for a device buffer at `dev`: writes them to the
+// host staging buffer `host` (same layout) and sets d_jobs / d_tiles.
+void plan_pack(std::vector<ClassPlan> &classes, uint8_t *host, uint8_t *dev);
+// One launch of a class kernel: a grid of wg_per_cu * ncu workgroups (all
+// tiles when wg_per_cu is 0) striding over the class's tiles.
+int plan_launch_class(const ClassPlan &c, int wg_per_cu, int ncu, hipStream_t st);
+
+}  // namespace s2tc

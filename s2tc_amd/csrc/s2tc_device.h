@@ -51,7 +51,9 @@ template <int CD> struct Dist;
 // AVG: 4dr^2 + dg^2 + 4db^2  (:217-223)
 template <> struct Dist<CD_AVG> {
 	typedef FeatI F;
+	static const bool kCheap = true;  // featq is three bitfield extracts
 	static S2TC_D F feat(uint32_t v) { return F{c_r(v), c_g(v), c_b(v)}; }
+	static S2TC_D F featq(uint32_t t) { return F{(int)(t & 31u), (int)((t >> 8) & 63u), (int)((t >> 16) & 31u)}; }
 	static S2TC_D int d(const F &a, const F &b)
 	{
 		int dr = a.x - b.x, dg = a.y - b.y, db = a.z - b.z;
@@ -61,7 +63,9 @@ template <> struct Dist<CD_AVG> {
 // W0AVG: dr^2 + dg^2 + db^2  (:225-232)
 template <> struct Dist<CD_W0AVG> {
 	typedef FeatI F;
+	static const bool kCheap = true;  // featq is three bitfield extracts
 	static S2TC_D F feat(uint32_t v) { return F{c_r(v), c_g(v), c_b(v)}; }
+	static S2TC_D F featq(uint32_t t) { return F{(int)(t & 31u), (int)((t >> 8) & 63u), (int)((t >> 16) & 31u)}; }
 	static S2TC_D int d(const F &a, const F &b)
 	{
 		int dr = a.x - b.x, dg = a.y - b.y, db = a.z - b.z;
@@ -71,7 +75,9 @@ template <> struct Dist<CD_W0AVG> {
 // WAVG: 4dr^2 + 4dg^2 + db^2  (:234-241)
 template <> struct Dist<CD_WAVG> {
 	typedef FeatI F;
+	static const bool kCheap = true;  // featq is three bitfield extracts
 	static S2TC_D F feat(uint32_t v) { return F{c_r(v), c_g(v), c_b(v)}; }
+	static S2TC_D F featq(uint32_t t) { return F{(int)(t & 31u), (int)((t >> 8) & 63u), (int)((t >> 16) & 31u)}; }
 	static S2TC_D int d(const F &a, const F &b)
 	{
 		int dr = a.x - b.x, dg = a.y - b.y, db = a.z - b.z;
@@ -82,7 +88,9 @@ template <> struct Dist<CD_WAVG> {
 // (|y| <= 6259*31, so every term stays far below 2^31).
 template <int KR, int KG, int KB> struct DistYuvLike {
 	typedef FeatI F;
+	static const bool kCheap = true;
 	static S2TC_D F feat(uint32_t v) { return F{c_r(v), c_g(v), c_b(v)}; }
+	static S2TC_D F featq(uint32_t t) { return F{(int)(t & 31u), (int)((t >> 8) & 63u), (int)((t >> 16) & 31u)}; }
 	static S2TC_D int d(const F &a, const F &b)
 	{
 		int dr = a.x - b.x, dg = a.y - b.y, db = a.z - b.z;
@@ -99,6 +107,8 @@ template <> struct Dist<CD_RGB> : DistYuvLike<42, 72, 14> {};
 // products go through uint32.  Not symmetric in its arguments.
 template <> struct Dist<CD_SRGB> {
 	typedef FeatI F;
+	static const bool kCheap = false;  // features are kept per texel
+	static S2TC_D F featq(uint32_t t) { return feat(((t & 0x1fu) << 11) | (((t >> 8) & 0x3fu) << 5) | ((t >> 16) & 0x1fu)); }
 	static S2TC_D F feat(uint32_t v)
 	{
 		int r = c_r(v), g = c_g(v), b = c_b(v);
@@ -126,6 +136,8 @@ S2TC_D int srgb_get_y(int r, int g, int b)
 // SRGB_MIXED (:298-315): feature (Y, 191r - Y, 191b - Y).
 template <> struct Dist<CD_SRGB_MIXED> {
 	typedef FeatI F;
+	static const bool kCheap = false;  // features are kept per texel
+	static S2TC_D F featq(uint32_t t) { return feat(((t & 0x1fu) << 11) | (((t >> 8) & 0x3fu) << 5) | ((t >> 16) & 0x1fu)); }
 	static S2TC_D F feat(uint32_t v)
 	{
 		int r = c_r(v), g = c_g(v), b = c_b(v);
@@ -142,6 +154,8 @@ template <> struct Dist<CD_SRGB_MIXED> {
 // reference's operation order; distance = (int)(100000 * sum (cb-ca)^2).
 template <> struct Dist<CD_NORMALMAP> {
 	typedef FeatF F;
+	static const bool kCheap = false;  // features are kept per texel
+	static S2TC_D F featq(uint32_t t) { return feat(((t & 0x1fu) << 11) | (((t >> 8) & 0x3fu) << 5) | ((t >> 16) & 0x1fu)); }
 	static S2TC_D F feat(uint32_t v)
 	{
 		float x = (float)c_r(v) / 31.0f * 2.0f - 1.0f;
@@ -164,6 +178,24 @@ template <> struct Dist<CD_NORMALMAP> {
 };
 
 template <int CD> struct IsSymmetric { static const bool value = true; };
+
+// Features of the 16 texels of a block: kept in registers for distances
+// with expensive features, recomputed from the texel word otherwise (which
+// saves 48 VGPRs and keeps occupancy up).
+template <int CD, bool STORE = !Dist<CD>::kCheap> struct TexFeat;
+template <int CD> struct TexFeat<CD, true> {
+	typename Dist<CD>::F f[16];
+	S2TC_D void init(const uint32_t (&tx)[16])
+	{
+#pragma unroll
+		for (int k = 0; k < 16; ++k) f[k] = Dist<CD>::featq(tx[k]);
+	}
+	S2TC_D typename Dist<CD>::F get(const uint32_t (&)[16], int k) const { return f[k]; }
+};
+template <int CD> struct TexFeat<CD, false> {
+	S2TC_D void init(const uint32_t (&)[16]) {}
+	S2TC_D typename Dist<CD>::F get(const uint32_t (&tx)[16], int k) const { return Dist<CD>::featq(tx[k]); }
+};
 template <> struct IsSymmetric<CD_SRGB> { static const bool value = false; };
 
 // alpha_dist (:358-361)
