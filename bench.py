@@ -99,6 +99,67 @@ def cpu_baseline(items, budget_s=12.0):
                        f"DITHER_NONE quantization + the reference's block loop, {n} threads, {t_used:.1f} s")
 
 
+class HipEvents:
+    """HIP events on the null stream (the stream encode_image_device launches on)."""
+    def __init__(self):
+        try:
+            self.h = ctypes.CDLL("libamdhip64.so")
+        except OSError:
+            self.h = ctypes.CDLL("/opt/rocm/lib/libamdhip64.so")
+        self.ev = [ctypes.c_void_p(), ctypes.c_void_p()]
+        for e in self.ev:
+            assert self.h.hipEventCreate(ctypes.byref(e)) == 0
+
+    def record(self, i):
+        assert self.h.hipEventRecord(self.ev[i], None) == 0
+
+    def elapsed_ms(self):
+        assert self.h.hipEventSynchronize(self.ev[1]) == 0
+        ms = ctypes.c_float()
+        assert self.h.hipEventElapsedTime(ctypes.byref(ms), self.ev[0], self.ev[1]) == 0
+        return ms.value
+
+    def close(self):
+        for e in self.ev:
+            self.h.hipEventDestroy(e)
+
+
+def quick_leg(s, size=16384, reps=10):
+    """Quick mode (MODE_FAST) on one large RGBA8 image in HBM: the sinusoid +
+    noise content of the 256x256 case at size x size, WAVG, REFINE_ALWAYS,
+    DITHER_NONE quantization on load; k_fast launches timed with HIP events.
+    Algorithmic bytes: 4 per texel read + 8 (DXT1) or 16 (DXT5) per block
+    written."""
+    from s2tc_amd import synth
+    src = synth.sinus(size, size, 0)
+    d_src = s.DeviceBuffer.from_host(src)
+    del src
+    ev = HipEvents()
+    out = []
+    for dxt in (0, 2):
+        nb = s.nblocks(size, size)
+        d_dst = s.DeviceBuffer(nb * s.block_bytes(dxt))
+        s.encode_image_device(d_src, s.SRC_RGBA8, size, size, d_dst, dxt, s.WAVG, -1, s.REFINE_ALWAYS)
+        s.synchronize()
+        ev.record(0)
+        for _ in range(reps):
+            s.encode_image_device(d_src, s.SRC_RGBA8, size, size, d_dst, dxt, s.WAVG, -1, s.REFINE_ALWAYS)
+        ev.record(1)
+        ms = ev.elapsed_ms() / reps
+        texels = size * size
+        nbytes = 4 * texels + nb * s.block_bytes(dxt)
+        gbs = nbytes / (ms * 1e-3) / 1e9
+        out.append(dict(kernel=f"k_fast<WAVG, {('DXT1', 'DXT3', 'DXT5')[dxt]}, rgba8>", image=f"{size}x{size}",
+                        dxt=(1, 3, 5)[dxt], texels=texels, kernel_ms=round(ms, 4),
+                        mtexels_s=round(texels / (ms * 1e-3) / 1e6, 1), algorithmic_bytes=nbytes,
+                        achieved_gbs=round(gbs, 1), frac=round(gbs / HBM_PEAK_GBS, 4)))
+        log(f"quick {out[-1]}")
+        d_dst.free()
+    ev.close()
+    d_src.free()
+    return out
+
+
 def serve_leg(s, rates, duration, npool=256, seed=4):
     """Open-loop Poisson arrivals of textures (with mip chains) drawn from a
     pool of the serving mix (70% quick mode, 30% nrandom 64 + LOOP), sources
@@ -147,6 +208,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-quick", action="store_true")
+    ap.add_argument("--quick-size", type=int, default=16384)
     ap.add_argument("--serve-duration", type=float, default=0.5)
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="only run a plain device copy of this many MiB (PMC calibration) and exit")
@@ -212,6 +275,14 @@ def main():
     roofline = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kname,
                 "kernel_ms": round(dom["ms"], 4), "algorithmic_bytes": dom["bytes"]}
+    if dom["normal"] == 2:
+        # pair-scan floor of reduce_colors_inplace: n m (m - 1) min+add lane-ops
+        # per block with n = m = 16, twice for DXT5 alpha
+        ops = 16 * 16 * 15 * (2 if dom["dxt"] == 2 else 1) * dom["blocks"]
+        tops = ops / (dom["ms"] * 1e-3) / 1e12
+        roofline["valu"] = {"bound": "valu", "achieved": round(tops, 3), "peak": VALU_PEAK_TOPS,
+                            "unit": "Tlane-op/s", "frac": round(tops / VALU_PEAK_TOPS, 4),
+                            "algorithmic_ops": ops}
     class_summary = [dict(kernel={0: "fast", 1: "normal", 2: "all16"}[c["normal"]], cd=names[c["cd"]], dxt=c["dxt"],
                           jobs=c["njobs"], texels=c["texels"], ms=round(c["ms"], 4),
                           gbs=round(c["bytes"] / (c["ms"] * 1e-3) / 1e9, 1)) for c in classes]
@@ -228,6 +299,10 @@ def main():
         pdt = (time.perf_counter() - t0) / k2
         plain = dict(ms_per_step=round(pdt * 1e3, 3), value=round(texels / pdt / 1e6, 1),
                      launches=len(items), speedup_batched=round(pdt / dt, 2))
+
+    quick = None
+    if not a.no_quick and a.workload == "batch":
+        quick = quick_leg(s, a.quick_size)
 
     serve = None
     if not a.no_serve and a.workload == "batch":
@@ -251,6 +326,7 @@ def main():
         "plain_per_texture": plain,
         "roofline": roofline,
         "classes": class_summary,
+        "quick_mode": quick,
         "cpu_baseline": cpu,
         "serve": serve,
     }

@@ -22,15 +22,57 @@ namespace s2tc {
 // Upper-triangle index of (i, j), i < j < 16.
 __host__ __device__ constexpr int tri(int i, int j) { return i * 16 - i * (i + 1) / 2 + (j - i - 1); }
 
-// Argmin of the pair scan; D(i, k) = dists[i][k] of the reference (symmetric
-// storage of the square: dists[i][k] = dist(c[min], c[max]), diagonal 0).
-// use: bit k set if slot k is a candidate.  Returns the pair as (bi, bj).
+// Upper bound of one distance value, for modes whose 16-term pair sums stay
+// below 2^23 (0: no such bound is used).  NORMALMAP: 100000 * |b - a|^2 of
+// two unit vectors (plus float slack); quadratic modes: w . (31, 63, 31)^2.
+template <int CD> struct DistBound { static const uint32_t value = 0; };
+template <> struct DistBound<CD_NORMALMAP> { static const uint32_t value = 400100; };
+template <> struct DistBound<CD_AVG> { static const uint32_t value = 11657; };
+template <> struct DistBound<CD_W0AVG> { static const uint32_t value = 5891; };
+template <> struct DistBound<CD_WAVG> { static const uint32_t value = 20681; };
+
+// D(i, k) = dists[i][k] of the reference, stored as the upper triangle of the
+// symmetric square (diagonal 0).
+S2TC_D int dpair(const int (&D)[120], int i, int k) { return k == i ? 0 : D[k < i ? tri(k, i) : tri(i, k)]; }
+
+// Argmin of the pair scan of reduce_colors_inplace (:363-414) for bounded
+// non-negative distances, as one unsigned min over keys
+// (sum << 8 | i << 4 | j): the smallest sum, then the first pair in the
+// reference's (i, j) order, which is its "sum < bestsum" rule.  Terms with
+// k == i or k == j are min(0, .) = 0 and are skipped.  A pair with a slot
+// outside the candidate set starts at 2^23, above every real sum (< 2^23),
+// so it never wins.  F: the 2-fixpoint variant (:415-478), every term also
+// bounded by f[k].  Returns (i << 4 | j).
+template <bool F>
+S2TC_D uint32_t fold_pairs_key(const int (&D)[120], const int (&f)[16], uint32_t use)
+{
+	uint32_t best = 0xffffffffu;
+#pragma unroll
+	for (int i = 0; i < 16; ++i) {
+#pragma unroll
+		for (int j = i + 1; j < 16; ++j) {
+			uint32_t s = ((use >> i) & (use >> j) & 1u) ? 0u : (1u << 23);
+#pragma unroll
+			for (int k = 0; k < 16; ++k) {
+				if (k == i || k == j) continue;
+				int t = min(dpair(D, i, k), dpair(D, j, k));
+				if (F) t = min(t, f[k]);
+				s += (uint32_t)t;
+			}
+			best = min(best, (s << 8) | (uint32_t)(i << 4 | j));
+		}
+	}
+	return best & 255u;
+}
+
+// Argmin of the pair scan for any distance, with the reference's own update
+// rule "bestsum < 0 || sum < bestsum" (int32-wrapped negative SRGB sums
+// included).  use: bit k set if slot k is a candidate.  Returns (i << 4 | j).
 template <bool NONNEG>
-S2TC_D void fold_pairs(const int (&D)[120], uint32_t use, int &bi, int &bj)
+S2TC_D uint32_t fold_pairs(const int (&D)[120], uint32_t use)
 {
 	int bestsum = -1;
-	bi = 0;
-	bj = 1;
+	uint32_t best = 1u;
 #pragma unroll
 	for (int i = 0; i < 16; ++i) {
 #pragma unroll
@@ -38,46 +80,17 @@ S2TC_D void fold_pairs(const int (&D)[120], uint32_t use, int &bi, int &bj)
 			int s = 0;
 #pragma unroll
 			for (int k = 0; k < 16; ++k) {
-				const int di = k == i ? 0 : D[k < i ? tri(k, i) : tri(i, k)];
-				const int dj = k == j ? 0 : D[k < j ? tri(k, j) : tri(j, k)];
+				if (NONNEG && (k == i || k == j)) continue;
+				const int di = dpair(D, i, k), dj = dpair(D, j, k);
 				s = (int)((uint32_t)s + (uint32_t)(di < dj ? di : dj));
 			}
 			const bool ok = ((use >> i) & 1u) && ((use >> j) & 1u);
 			const bool take = ok && (NONNEG ? (uint32_t)s < (uint32_t)bestsum : (bestsum < 0 || s < bestsum));
 			bestsum = take ? s : bestsum;
-			bi = take ? i : bi;
-			bj = take ? j : bj;
+			best = take ? (uint32_t)(i << 4 | j) : best;
 		}
 	}
-}
-
-// As fold_pairs for reduce_colors_inplace_2fixpoints (:415-478): every term
-// is also bounded by f[k] = min(dist(0, a_k), dist(255, a_k)).  Alpha
-// distances are never negative.
-S2TC_D void fold_pairs_2fix(const int (&A)[120], const int (&f)[16], uint32_t use, int &bi, int &bj)
-{
-	uint32_t bestsum = 0xffffffffu;
-	bi = 0;
-	bj = 1;
-#pragma unroll
-	for (int i = 0; i < 16; ++i) {
-#pragma unroll
-		for (int j = i + 1; j < 16; ++j) {
-			uint32_t s = 0;
-#pragma unroll
-			for (int k = 0; k < 16; ++k) {
-				const int di = k == i ? 0 : A[k < i ? tri(k, i) : tri(i, k)];
-				const int dj = k == j ? 0 : A[k < j ? tri(k, j) : tri(j, k)];
-				int t = di < dj ? di : dj;
-				t = t < f[k] ? t : f[k];
-				s += (uint32_t)t;
-			}
-			const bool take = ((use >> i) & 1u) && ((use >> j) & 1u) && s < bestsum;
-			bestsum = take ? s : bestsum;
-			bi = take ? i : bi;
-			bj = take ? j : bj;
-		}
-	}
+	return best;
 }
 
 template <int CD, int DXT>
@@ -128,7 +141,16 @@ S2TC_D BlockOut encode_block_all16(const uint32_t (&tx)[16], int w, int h, int r
 					for (int j = i + 1; j < 16; ++j)
 						D[tri(i, j)] = (((use >> i) & (use >> j)) & 1u) ? Dd::d(f[i], f[j]) : 0;
 			}
-			fold_pairs<IsSymmetric<CD>::value>(D, use, bi, bj);
+			__builtin_amdgcn_sched_barrier(0);
+			uint32_t p;
+			if constexpr (DistBound<CD>::value != 0) {
+				const int none[16] = {};
+				p = fold_pairs_key<false>(D, none, use);
+			} else {
+				p = fold_pairs<IsSymmetric<CD>::value>(D, use);
+			}
+			bi = (int)(p >> 4);
+			bj = (int)(p & 15u);
 		}
 		uint32_t v0 = 0, v1 = 0;
 #pragma unroll
@@ -139,6 +161,8 @@ S2TC_D BlockOut encode_block_all16(const uint32_t (&tx)[16], int w, int h, int r
 		c0 = v0;
 		c1 = v1;
 		if (DXT == DXT5) {
+			// keep the alpha triangle out of the colour triangle's live range
+			__builtin_amdgcn_sched_barrier(0);
 			int ai, aj;
 			{
 				int A[120], fx[16];
@@ -154,7 +178,9 @@ S2TC_D BlockOut encode_block_all16(const uint32_t (&tx)[16], int w, int h, int r
 					for (int j = i + 1; j < 16; ++j)
 						A[tri(i, j)] =
 						    (((valid >> i) & (valid >> j)) & 1u) ? adist(texel_a(tx[i]), texel_a(tx[j])) : 0;
-				fold_pairs_2fix(A, fx, valid, ai, aj);
+				const uint32_t p = fold_pairs_key<true>(A, fx, valid);
+				ai = (int)(p >> 4);
+				aj = (int)(p & 15u);
 			}
 #pragma unroll
 			for (int k = 0; k < 16; ++k) {
@@ -163,6 +189,7 @@ S2TC_D BlockOut encode_block_all16(const uint32_t (&tx)[16], int w, int h, int r
 			}
 		}
 	}
+	__builtin_amdgcn_sched_barrier(0);
 	TexFeat<CD> f;
 	f.init(tx);
 	const Masks mk = block_masks<DXT>(tx, w, h);
@@ -188,7 +215,7 @@ S2TC_D BlockOut encode_block_all16(const uint32_t (&tx)[16], int w, int h, int r
 	} else {
 		a1 = fix_equal_a(a0, a1);
 		const uint32_t bits = color_refine_lane<CD, false, false>(tx, f, mk, refine, c0, c1);
-		const uint64_t ab = alpha_refine_lane<false>(tx, mk, refine, a0, a1);
+		const uint64_t ab = alpha_refine_fast<false>(tx, mk.valid, refine, a0, a1);
 		const uint64_t lo = (uint64_t)(a0 & 255) | ((uint64_t)(a1 & 255) << 8) | (ab << 16);
 		o.w[0] = (uint32_t)lo;
 		o.w[1] = (uint32_t)(lo >> 32);

@@ -204,6 +204,70 @@ S2TC_D uint64_t alpha_refine_lane(const uint32_t (&tx)[16], const Masks &mk, int
 	return bits;
 }
 
+// The alpha pass of alpha_try_lane without its score, in threshold form.
+// For endpoints e0, e1 in [0, 255] and a texel alpha a (all integers):
+//   a^2 <= (a - e)^2         <=>  e == 0   || 2a <= e
+//   (255 - a)^2 <= (a - e)^2 <=>  e == 255 || 2a >= 255 + e
+//   (a - e1)^2 < (a - e0)^2  <=>  (e1 - e0) (2a - e0 - e1) > 0
+// so "take 0", "take 255" and "endpoint 1 is nearer" are comparisons of a
+// against per-pass values.  Counts and sums travel packed as n << 16 | sum
+// (sum <= 16 * 255).
+template <bool FULL>
+S2TC_D uint64_t alpha_try_thr(const uint32_t (&tx)[16], uint32_t valid, int a0, int a1, int &n0, int &s0, int &n1,
+                              int &s1)
+{
+	const int zh = min(a0 == 0 ? 1023 : a0, a1 == 0 ? 1023 : a1) >> 1;                // 0 wins  <=>  a <= zh
+	const int fc = (max(a0 == 255 ? 0 : 255 + a0, a1 == 255 ? 0 : 255 + a1) + 1) >> 1;  // 255 wins <=> a >= fc
+	const int D2 = (a1 - a0) * 2, SD = (a0 + a1) * (a1 - a0);                           // e1 nearer <=> a D2 > SD
+	uint32_t lo = 0, hi = 0, cacc = 0, c1 = 0;
+#pragma unroll
+	for (int k = 0; k < 16; ++k) {
+		const int x = k >> 2, y = k & 3, slot = y * 4 + x;
+		const bool v = FULL || ((valid >> k) & 1u);
+		const int a = texel_a(tx[k]);
+		const bool z = a <= zh, full = !z && a >= fc, b = a * D2 > SD;
+		const bool acc = v && !z && !full;
+		const uint32_t idx = v ? (z ? 6u : full ? 7u : (b ? 1u : 0u)) : 0u;
+		if (3 * slot + 2 < 32) {
+			lo |= idx << (3 * slot);
+		} else if (3 * slot >= 32) {
+			hi |= idx << (3 * slot - 32);
+		} else {
+			lo |= idx << (3 * slot);
+			hi |= idx >> (32 - 3 * slot);
+		}
+		const uint32_t c = (uint32_t)a | 0x10000u;
+		cacc += acc ? c : 0u;
+		c1 += (acc && b) ? c : 0u;
+	}
+	n1 = (int)(c1 >> 16);
+	s1 = (int)(c1 & 0xffffu);
+	n0 = (int)(cacc >> 16) - n1;
+	s0 = (int)(cacc & 0xffffu) - s1;
+	return ((uint64_t)hi << 32) | lo;
+}
+
+// DXT5 alpha refinement with the threshold pass for NEVER and ALWAYS (which
+// need no score); LOOP keeps the scoring pass.
+template <bool FULL>
+S2TC_D uint64_t alpha_refine_fast(const uint32_t (&tx)[16], uint32_t valid, int refine, int &a0, int &a1)
+{
+	int n0, s0, n1, s1;
+	if (refine == REFINE_NEVER) {
+		if (a1 < a0) {
+			int t = a0;
+			a0 = a1;
+			a1 = t;
+		}
+		return alpha_try_thr<FULL>(tx, valid, a0, a1, n0, s0, n1, s1);
+	}
+	if (refine == REFINE_LOOP) return alpha_refine_lane<FULL>(tx, Masks{valid, 0u}, refine, a0, a1);
+	uint64_t bits = alpha_try_thr<FULL>(tx, valid, a0, a1, n0, s0, n1, s1);
+	eval_alpha(n0, s0, n1, s1, a0, a1);
+	alpha_fix_order(bits, a0, a1);
+	return bits;
+}
+
 // colour refinement NEVER/ALWAYS/LOOP (:767-860) on the lane's texels
 template <int CD, bool TRANS, bool FULL>
 S2TC_D uint32_t color_refine_lane(const uint32_t (&tx)[16], const TexFeat<CD> &f, const Masks &mk, int refine,
@@ -303,7 +367,7 @@ S2TC_D BlockOut encode_block_fast_t(const uint32_t (&tx)[16], int w, int h, int 
 	} else {
 		a1 = fix_equal_a(a0, a1);
 		const uint32_t bits = color_refine_lane<CD, false, FULL>(tx, f, mk, refine, c0, c1);
-		const uint64_t ab = alpha_refine_lane<FULL>(tx, mk, refine, a0, a1);
+		const uint64_t ab = alpha_refine_fast<FULL>(tx, mk.valid, refine, a0, a1);
 		const uint64_t lo = (uint64_t)(a0 & 255) | ((uint64_t)(a1 & 255) << 8) | (ab << 16);
 		o.w[0] = (uint32_t)lo;
 		o.w[1] = (uint32_t)(lo >> 32);

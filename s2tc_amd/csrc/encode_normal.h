@@ -31,8 +31,12 @@ namespace s2tc {
 
 #define S2TC_RS 20  // LDS row stride in ints (16 values + 4 pad: 16-byte aligned, fewer bank conflicts)
 
-template <int CTRL> S2TC_D int dpp_i(int v) { return __builtin_amdgcn_mov_dpp(v, CTRL, 0xf, 0xf, false); }
-template <int CTRL> S2TC_D uint32_t dpp_u(uint32_t v) { return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, CTRL, 0xf, 0xf, false); }
+// Every control used here (quad_perm, row mirrors, row_ror) reads a lane of
+// the same row, and rows are all-active or all-inactive, so bound_ctrl never
+// applies; the update_dpp form with bound_ctrl lets the compiler fold the
+// lane move into the consuming VALU op (v_min_i32_dpp, v_add_u32_dpp).
+template <int CTRL> S2TC_D int dpp_i(int v) { return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xf, 0xf, true); }
+template <int CTRL> S2TC_D uint32_t dpp_u(uint32_t v) { return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xf, 0xf, true); }
 
 // all-reduce within a DPP row of 16 lanes: quad_perm [1,0,3,2], quad_perm
 // [2,3,0,1], row_half_mirror, row_mirror

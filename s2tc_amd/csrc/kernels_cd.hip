@@ -6,6 +6,7 @@
 // parallel.
 #include "encode_all16.h"
 #include "encode_normal.h"
+#include "encode_quad.h"
 #include "kernel_table.h"
 
 #ifndef S2TC_CD
@@ -75,8 +76,15 @@ __global__ __launch_bounds__(256) void k_fast(const uint32_t *__restrict__ src, 
 	const int by = b / bw, bx = b - by * bw;
 	const int w = min(4, width - 4 * bx), h = min(4, height - 4 * by);
 	uint32_t tx[16];
-	load_block<DXT, RAW>(src, width, bx, by, w, h, vec != 0, tx);
-	const BlockOut o = encode_block_fast<CD, DXT>(tx, w, h, refine);
+	BlockOut o;
+	if constexpr (QuadW<CD>::value) {
+		load_block<DXT, false>(src, width, bx, by, w, h, vec != 0, tx);
+		const uint32_t *blk = src + (size_t)(4 * by) * (size_t)width + 4 * bx;
+		o = encode_block_quad_any<CD, DXT, RAW>(tx, blk, width, w, h, refine);
+	} else {
+		load_block<DXT, RAW>(src, width, bx, by, w, h, vec != 0, tx);
+		o = encode_block_fast<CD, DXT>(tx, w, h, refine);
+	}
 	store_block<DXT>(dst, (size_t)b, o);
 }
 
@@ -162,9 +170,17 @@ __global__ __launch_bounds__(256) void k_fast_batch(const JobDesc *__restrict__ 
 		const int w = min(4, j.width - 4 * bx), h = min(4, j.height - 4 * by);
 		const bool vec = (j.width & 3) == 0;
 		uint32_t tx[16];
-		if (j.raw) load_block<DXT, true>(j.src, j.width, bx, by, w, h, vec, tx);
-		else load_block<DXT, false>(j.src, j.width, bx, by, w, h, vec, tx);
-		const BlockOut o = encode_block_fast<CD, DXT>(tx, w, h, j.refine);
+		BlockOut o;
+		if constexpr (QuadW<CD>::value) {
+			load_block<DXT, false>(j.src, j.width, bx, by, w, h, vec, tx);
+			const uint32_t *blk = j.src + (size_t)(4 * by) * (size_t)j.width + 4 * bx;
+			if (j.raw) o = encode_block_quad_any<CD, DXT, true>(tx, blk, j.width, w, h, j.refine);
+			else o = encode_block_quad_any<CD, DXT, false>(tx, blk, j.width, w, h, j.refine);
+		} else {
+			if (j.raw) load_block<DXT, true>(j.src, j.width, bx, by, w, h, vec, tx);
+			else load_block<DXT, false>(j.src, j.width, bx, by, w, h, vec, tx);
+			o = encode_block_fast<CD, DXT>(tx, w, h, j.refine);
+		}
 		store_block<DXT>(j.dst, (size_t)b, o);
 	}
 }
@@ -228,6 +244,23 @@ __global__ __launch_bounds__(64) void k_normal_batch(const JobDesc *__restrict__
 	}
 }
 
+#ifdef S2TC_SRGB_Y_TABLE
+__global__ __launch_bounds__(256) void k_srgb_y_table_init(int)
+{
+	const uint32_t v = blockIdx.x * 256u + threadIdx.x;
+	s2tc_srgb_y_table[v] = (uint16_t)srgb_get_y(c_r(v), c_g(v), c_b(v));
+}
+#endif
+
+#ifdef S2TC_NM_TABLE
+__global__ __launch_bounds__(256) void k_nm_table_init(int)
+{
+	const uint32_t v = blockIdx.x * 256u + threadIdx.x;
+	const FeatF f = Dist<CD_NORMALMAP>::compute(v);
+	s2tc_nm_table[v] = make_float4(f.x, f.y, f.z, 0.0f);
+}
+#endif
+
 }  // namespace s2tc
 
 using namespace s2tc;
@@ -266,5 +299,12 @@ extern "C" void S2TC_CAT(s2tc_kernels_cd, S2TC_CD)(KernelTable *kt)
 	kt->all16_batch[DXT5] = (const void *)k_all16_batch<S2TC_CD, DXT5>;
 	kt->normal[0] = (const void *)k_normal<S2TC_CD, false>;
 	kt->normal[1] = (const void *)k_normal<S2TC_CD, true>;
+#if defined(S2TC_NM_TABLE)
+	kt->init = (const void *)k_nm_table_init;
+#elif defined(S2TC_SRGB_Y_TABLE)
+	kt->init = (const void *)k_srgb_y_table_init;
+#else
+	kt->init = nullptr;
+#endif
 	kt->normal_smem_per_cand = (int)(4 * (sizeof(uint32_t) + sizeof(Dist<S2TC_CD>::F) + S2TC_RS * sizeof(int)));
 }

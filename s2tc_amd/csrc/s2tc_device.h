@@ -133,16 +133,31 @@ S2TC_D int srgb_get_y(int r, int g, int b)
 	int y = 37 * (r * r * 84 + g * g * 72 + b * b * 28);
 	return (int)(sqrtf((float)y) + 0.5f);
 }
+
+#if defined(S2TC_CD) && S2TC_CD == 3
+// srgb_get_y of all 65536 RGB565 colours (at most 3820), filled once per
+// device by k_srgb_y_table_init; only the SRGB_MIXED kernel object defines it.
+__device__ uint16_t s2tc_srgb_y_table[65536];
+#define S2TC_SRGB_Y_TABLE 1
+#endif
+
 // SRGB_MIXED (:298-315): feature (Y, 191r - Y, 191b - Y).
 template <> struct Dist<CD_SRGB_MIXED> {
 	typedef FeatI F;
 	static const bool kCheap = false;  // features are kept per texel
 	static S2TC_D F featq(uint32_t t) { return feat(((t & 0x1fu) << 11) | (((t >> 8) & 0x3fu) << 5) | ((t >> 16) & 0x1fu)); }
+	static S2TC_D int luma(uint32_t v)
+	{
+#ifdef S2TC_SRGB_Y_TABLE
+		return (int)s2tc_srgb_y_table[v & 0xffffu];
+#else
+		return srgb_get_y(c_r(v), c_g(v), c_b(v));
+#endif
+	}
 	static S2TC_D F feat(uint32_t v)
 	{
-		int r = c_r(v), g = c_g(v), b = c_b(v);
-		int y = srgb_get_y(r, g, b);
-		return F{y, r * 191 - y, b * 191 - y};
+		const int y = luma(v);
+		return F{y, c_r(v) * 191 - y, c_b(v) * 191 - y};
 	}
 	static S2TC_D int d(const F &a, const F &b)
 	{
@@ -150,13 +165,21 @@ template <> struct Dist<CD_SRGB_MIXED> {
 		return ((y * y) << 3) + shrr(u * u, 1) + shrr(v * v, 2);
 	}
 };
+#if defined(S2TC_CD) && S2TC_CD == 7
+// NORMALMAP features of all 65536 RGB565 colours, filled once per device by
+// k_nm_table_init with Dist<CD_NORMALMAP>::compute, so a lookup returns the
+// very floats the formula gives.  Only the NORMALMAP kernel object defines it.
+__device__ float4 s2tc_nm_table[65536];
+#define S2TC_NM_TABLE 1
+#endif
+
 // NORMALMAP (:317-354): feature = the normalised vector, computed with the
 // reference's operation order; distance = (int)(100000 * sum (cb-ca)^2).
 template <> struct Dist<CD_NORMALMAP> {
 	typedef FeatF F;
 	static const bool kCheap = false;  // features are kept per texel
 	static S2TC_D F featq(uint32_t t) { return feat(((t & 0x1fu) << 11) | (((t >> 8) & 0x3fu) << 5) | ((t >> 16) & 0x1fu)); }
-	static S2TC_D F feat(uint32_t v)
+	static S2TC_D F compute(uint32_t v)
 	{
 		float x = (float)c_r(v) / 31.0f * 2.0f - 1.0f;
 		float y = (float)c_g(v) / 63.0f * 2.0f - 1.0f;
@@ -170,6 +193,15 @@ template <> struct Dist<CD_NORMALMAP> {
 		}
 		return F{x, y, z};
 	}
+#ifdef S2TC_NM_TABLE
+	static S2TC_D F feat(uint32_t v)
+	{
+		const float4 t = s2tc_nm_table[v & 0xffffu];
+		return F{t.x, t.y, t.z};
+	}
+#else
+	static S2TC_D F feat(uint32_t v) { return compute(v); }
+#endif
 	static S2TC_D int d(const F &a, const F &b)
 	{
 		float dx = b.x - a.x, dy = b.y - a.y, dz = b.z - a.z;
@@ -201,21 +233,23 @@ template <> struct IsSymmetric<CD_SRGB> { static const bool value = false; };
 // alpha_dist (:358-361)
 S2TC_D int adist(int a, int b) { return (a - b) * (a - b); }
 
-// Unsigned x / d for 0 <= x < 2^20 and 1 <= d <= 64: float reciprocal
-// estimate, then one exact integer correction (the estimate is never above
-// the true quotient by a whole unit for these ranges).
-S2TC_D uint32_t udiv_small(uint32_t x, uint32_t d)
-{
-	float fq = (float)x * __builtin_amdgcn_rcpf((float)d);
-	uint32_t q = (uint32_t)fq;
-	int r = (int)(x - q * d);
-	q = r < 0 ? q - 1 : q;
-	r = r < 0 ? r + (int)d : r;
-	q += (uint32_t)r >= d ? 1u : 0u;
-	return q;
-}
-// ((S << 1) + n) / (n << 1): round-half-up mean (:547-549)
-S2TC_D int rmean(int s, int n) { return (int)udiv_small((uint32_t)((s << 1) + n), (uint32_t)(n << 1)); }
+// floor(x / d) for 0 <= x < 8192 and 1 <= d <= 32 as (x * m) >> 18 with
+// m = floor(2^18 / d) + 1: m - 2^18 / d lies in (0, 1], so the product
+// overshoots x / d by less than 8192 / 2^18 = 1/32 <= 1/d, and frac(x / d)
+// <= 1 - 1/d, so the floor never moves.  One 24-bit multiply per quotient.
+struct Recip {
+	uint32_t m;
+	S2TC_D explicit Recip(uint32_t d)
+	{
+		uint32_t q = (uint32_t)(262144.0f * __builtin_amdgcn_rcpf((float)d));
+		const int r = 262144 - (int)__umul24(q, d);  // exact correction of the estimate
+		q = r < 0 ? q - 1u : ((uint32_t)r >= d ? q + 1u : q);
+		m = q + 1u;
+	}
+	S2TC_D uint32_t div(uint32_t x) const { return __umul24(x, m) >> 18; }
+};
+// ((S << 1) + n) / (n << 1): round-half-up mean (:547-549); s <= 255 n, n <= 16
+S2TC_D int rmean(int s, int n) { return (int)Recip((uint32_t)(n << 1)).div((uint32_t)((s << 1) + n)); }
 
 // 2-bit colour index word fix-ups (:802-804, :810-812)
 S2TC_D uint32_t keep_ones(uint32_t bits)

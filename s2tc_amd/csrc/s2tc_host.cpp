@@ -12,6 +12,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <atomic>
 #include <mutex>
 
 namespace s2tc {
@@ -62,6 +63,36 @@ bool is_fast(int cd, int nrandom) { return cd != CD_NORMALMAP && nrandom < 0; }
 
 int normal_pbw(int nrandom) { return (16 + (nrandom > 0 ? nrandom : 0) + 15) / 16; }
 
+// Per-device tables (the NORMALMAP feature table) are built on the first
+// call that reaches a device, outside any stream capture: callers that
+// capture graphs make one ordinary call first.
+static std::mutex g_init_mu;
+static std::atomic<uint64_t> g_init_done{0};
+
+static int ensure_tables()
+{
+	int dev = 0;
+	if (int r = hip_check(hipGetDevice(&dev), "hipGetDevice")) return r;
+	const uint64_t bit = dev < 64 ? 1ull << dev : 0ull;
+	if (bit && (g_init_done.load(std::memory_order_acquire) & bit)) return 0;
+	std::lock_guard<std::mutex> lk(g_init_mu);
+	if (bit && (g_init_done.load(std::memory_order_acquire) & bit)) return 0;
+	hipStream_t st = nullptr;
+	if (int r = hip_check(hipStreamCreate(&st), "hipStreamCreate")) return r;
+	int rc = 0;
+	for (int cd = 0; cd < 8 && !rc; ++cd) {
+		const KernelTable &kt = kernel_table(cd);
+		if (!kt.init) continue;
+		int dummy = 0;
+		void *args[] = {&dummy};
+		rc = hip_check(hipLaunchKernel(kt.init, dim3(256), dim3(256), args, 0, st), "launch table set-up");
+	}
+	if (!rc) rc = hip_check(hipStreamSynchronize(st), "table set-up");
+	(void)hipStreamDestroy(st);
+	if (!rc && bit) g_init_done.fetch_or(bit, std::memory_order_acq_rel);
+	return rc;
+}
+
 int ensure_device()
 {
 	static int ok = -1;
@@ -71,7 +102,7 @@ int ensure_device()
 		ok = (e == hipSuccess && n > 0) ? 1 : 0;
 	}
 	if (!ok) return set_error(S2TC_E_NOGPU, "no HIP device available");
-	return 0;
+	return ensure_tables();
 }
 
 int launch_encode(const EncodeArgs &a, hipStream_t stream)

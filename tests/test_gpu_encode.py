@@ -74,3 +74,34 @@ def test_raw_rgba8_source(gpu, oracle):
             got = gpu.encode_image(rgba, w, h, dxt, cd, nr, refine, 1, src_kind=gpu.SRC_RGBA8)
             exp = oracle.encode_quantized(q, w, h, dxt, cd, nr, refine, 1)
             assert np.array_equal(got, exp), f"raw source mismatch {w}x{h} dxt={dxt} cd={cd}"
+
+
+def _rgba8_kinds(rng, w, h):
+    """RGBA8 images that stress the quick-mode paths: smooth content, uniform
+    noise (alpha on both sides of 128, so DXT1 blocks mix transparent and
+    opaque texels), flat blocks (all texels equal: equal endpoints), and
+    black/white/255-alpha extremes."""
+    yield "smooth", imgutil.smooth_rgba8(rng, w, h)
+    yield "noise", rng.integers(0, 256, w * h * 4).astype(np.uint8)
+    flat = np.empty((h, w, 4), np.uint8)
+    flat[...] = rng.integers(0, 256, 4).astype(np.uint8)
+    yield "flat", flat.reshape(-1)
+    ext = rng.choice(np.array([0, 255, 127, 128], np.uint8), w * h * 4)
+    yield "extreme", ext
+
+
+@pytest.mark.parametrize("cd", [4, 5, 6])
+def test_quick_quadratic_modes(gpu, oracle, cd):
+    """AVG, W0AVG and WAVG quick mode (dot-product form) on RGBA8 and on
+    quantized sources, every refine mode and format, full and partial blocks."""
+    rng = np.random.default_rng(300 + cd)
+    for (w, h), dxt in itertools.product([(32, 16), (13, 9), (6, 3), (1, 1)], range(3)):
+        for kind, rgba in _rgba8_kinds(rng, w, h):
+            q = oracle.rgb565_image(rgba, w, h, 4, imgutil.ALPHABITS[dxt], 0)
+            for refine in range(3):
+                _say(f"case quad {kind} {w}x{h} dxt={dxt} cd={cd} refine={refine}")
+                exp = oracle.encode_quantized(q, w, h, dxt, cd, -1, refine, 0)
+                got = gpu.encode_image(rgba, w, h, dxt, cd, -1, refine, 0, src_kind=gpu.SRC_RGBA8)
+                assert np.array_equal(got, exp), f"raw {kind} {w}x{h} dxt={dxt} cd={cd} refine={refine}"
+                got = gpu.encode_image(q, w, h, dxt, cd, -1, refine, 0)
+                assert np.array_equal(got, exp), f"quantized {kind} {w}x{h} dxt={dxt} cd={cd} refine={refine}"
