@@ -155,6 +155,21 @@ def quick_leg(s, size=16384, reps=10):
                         achieved_gbs=round(gbs, 1), frac=round(gbs / HBM_PEAK_GBS, 4)))
         log(f"quick {out[-1]}")
         d_dst.free()
+    # a plain streaming copy of the same source, for the achievable read+write rate
+    d_cp = s.DeviceBuffer(d_src.nbytes)
+    L = s.lib()
+    L.s2tc_amd_memcpy_d2d_kernel.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]
+    s.check(L.s2tc_amd_memcpy_d2d_kernel(d_cp.ptr, d_src.ptr, d_src.nbytes, None))
+    s.synchronize()
+    ev.record(0)
+    for _ in range(reps):
+        s.check(L.s2tc_amd_memcpy_d2d_kernel(d_cp.ptr, d_src.ptr, d_src.nbytes, None))
+    ev.record(1)
+    ms = ev.elapsed_ms() / reps
+    out.append(dict(kernel="k_copy16 (reference copy)", bytes=2 * d_src.nbytes, kernel_ms=round(ms, 4),
+                    achieved_gbs=round(2 * d_src.nbytes / (ms * 1e-3) / 1e9, 1)))
+    log(f"quick {out[-1]}")
+    d_cp.free()
     ev.close()
     d_src.free()
     return out

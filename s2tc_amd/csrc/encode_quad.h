@@ -8,15 +8,17 @@
 //
 // so "d1 < d0" is one signed 8-bit dot product per texel (v_dot4_i32_i8 of
 // the texel bytes against per-pass weight bytes, the threshold riding in its
-// accumulator input).  The pass score REFINE_LOOP compares is rebuilt
-// exactly from block sums (sum of |t|_w^2, the per-channel sums of all used
-// texels and of those that chose endpoint 1).  The darkest/brightest search
-// of MODE_FAST (:900-922) takes |t|_w^2 from v_dot4_u32_u8 and keeps
-// (distance << 4 | slot) keys in one v_max/v_min each, which is the
-// reference's first-occurrence rule; the two winning texels are then read
-// back from the source.  Everything is exact integer arithmetic, so the
-// output is bit-identical to evaluating the reference's distance texel by
-// texel.
+// accumulator), whose sign is the answer.  The signs of a texel row are
+// gathered into one word of 0/1 bytes with byte permutes; the cluster sums
+// that s2tc_evaluate_colors_result_t needs are then dot products of that
+// word with per-row byte planes of the channels (v_dot4_u32_u8), and the
+// 2-bit index slots of the row come from one multiply.  The pass score
+// REFINE_LOOP compares is rebuilt exactly from block sums.  The
+// darkest/brightest search of MODE_FAST (:900-922) takes |t|_w^2 from
+// v_dot4_u32_u8 and keeps (distance << 4 | slot) keys in one v_max/v_min
+// each, which is the reference's first-occurrence rule; the two winning
+// texels are read back from the source.  All of it is exact integer
+// arithmetic, bit-identical to evaluating the reference's distances.
 //
 // Texels arrive as source words: RGBA8 (RAW) or quantized
 // r5 | g6 << 8 | b5 << 16 | a << 24.
@@ -33,30 +35,26 @@ template <> struct QuadW<CD_AVG> { static const bool value = true; enum { R = 4,
 template <> struct QuadW<CD_W0AVG> { static const bool value = true; enum { R = 1, G = 1, B = 1, AG = 1 }; };
 template <> struct QuadW<CD_WAVG> { static const bool value = true; enum { R = 4, G = 4, B = 1, AG = 2 }; };
 
-// Per-texel quantities of the quadratic encoder, from one source word.
+// Per-texel quantities from one source word.
 template <int CD, bool RAW> struct QuadTexel {
 	typedef QuadW<CD> W;
-	uint32_t u;   // bytes (r, AG*g, b, 0)
-	int d;        // |t|_w^2 = d(t, black)
-	uint32_t ps;  // pack_sum: r | g << 10 | b << 21
+	uint32_t u;  // bytes (r, AG*g, b, 0)
+	int d;       // |t|_w^2 = d(t, black)
 	S2TC_D explicit QuadTexel(uint32_t s)
 	{
 		uint32_t rb;
 		if (RAW) {
-			const uint32_t l3 = s >> 3;
-			rb = l3 & 0x1f001fu;
+			rb = (s >> 3) & 0x1f001fu;
 			u = rb | (W::AG == 2 ? (s >> 1) & 0x7e00u : (s >> 2) & 0x3f00u);
-			ps = (l3 & 0x1fu) | (s & 0xfc00u) | ((s << 2) & 0x3e00000u);
 		} else {
 			rb = s & 0x1f001fu;
 			u = rb | (W::AG == 2 ? (s & 0x3f00u) << 1 : s & 0x3f00u);
-			ps = (s & 0x1fu) | ((s & 0x3f00u) << 2) | ((s & 0x1f0000u) << 5);
 		}
-		// v = the second dot operand: (w_r r, (w_g / AG) AG g, w_b b) / the texel's own factor
+		// second dot operand: bytes w_c t_c / (the texel byte's own factor)
 		uint32_t v;
-		if (CD == CD_AVG) v = u + 3u * rb;           // (4r, g, 4b)
-		else if (CD == CD_W0AVG) v = u;              // (r, g, b)
-		else v = u + 3u * (rb & 0x1fu);              // (4r, 2g, b) against (r, 2g, b)
+		if (CD == CD_AVG) v = u + 3u * rb;       // (4r, g, 4b)
+		else if (CD == CD_W0AVG) v = u;          // (r, g, b)
+		else v = u + 3u * (rb & 0x1fu);          // (4r, 2g, b) against (r, 2g, b)
 		d = (int)__builtin_amdgcn_udot4(u, v, 0u, false);
 	}
 	// DXT1: 0 for a transparent texel (quantized alpha 0), else all ones
@@ -78,15 +76,85 @@ S2TC_D int black565(int wr, int wg, int wb, uint32_t c)
 	return wr * r * r + wg * g * g + wb * b * b;
 }
 
-// One s2tc_try_encode_block pass for endpoints c0, c1 (565 words).  U: the
-// texels' dot bytes; ps: pack_sum of each used texel, 0 for texels that do
-// not count; use2: 0b01 in the 2-bit slot of every used texel.  Returns the
-// slot-ordered index bits of the used texels (1 = endpoint 1), n1 and s1 =
-// count and packed sum of those texels, and the pass score sum(min(d0, d1))
-// when SCORE.
+// Byte planes of the texels' dot bytes: r[y] byte x = r of texel (x, y),
+// likewise AG*g and b; use[y] byte x = 1 if texel (x, y) counts.
+struct QuadPlanes {
+	uint32_t r[4], g[4], b[4], use[4];
+};
+
+// Texel k (x = k >> 2, y = k & 3) -> planes, with byte permutes:
+// {S0, S1} selector bytes 0-3 pick S1's bytes, 4-7 S0's, 0x0c gives 0.
+S2TC_D void quad_planes(const uint32_t (&U)[16], QuadPlanes &P)
+{
+#pragma unroll
+	for (int y = 0; y < 4; ++y) {
+		const uint32_t a = __builtin_amdgcn_perm(U[4 + y], U[y], 0x05010400u);       // r0 r1 g0 g1
+		const uint32_t c = __builtin_amdgcn_perm(U[12 + y], U[8 + y], 0x05010400u);  // r2 r3 g2 g3
+		P.r[y] = __builtin_amdgcn_perm(c, a, 0x05040100u);
+		P.g[y] = __builtin_amdgcn_perm(c, a, 0x07060302u);
+		P.b[y] = __builtin_amdgcn_perm(U[4 + y], U[y], 0x0c0c0602u) | __builtin_amdgcn_perm(U[12 + y], U[8 + y], 0x06020c0cu);
+	}
+}
+
+// Bytes x of the result = sign bit (as 0/1) of v[x].
+S2TC_D uint32_t sign_bytes(int v0, int v1, int v2, int v3)
+{
+	const uint32_t lo = __builtin_amdgcn_perm((uint32_t)v1, (uint32_t)v0, 0x0c0c0703u);
+	const uint32_t hi = __builtin_amdgcn_perm((uint32_t)v3, (uint32_t)v2, 0x07030c0cu);
+	return ((lo | hi) >> 7) & 0x01010101u;
+}
+
+// 0/1 bytes x = 0..3 -> bits 2x (one multiply: byte x lands on bit 24 + 2x and
+// no other partial product reaches bits 24..31)
+S2TC_D uint32_t bytes_to_slots(uint32_t m) { return (m * 0x01041040u) >> 24; }
+
+struct QSums {
+	int r, g, b;  // sums of the dot bytes (r, AG*g, b)
+};
+
+S2TC_D QSums plane_sums(const QuadPlanes &P, const uint32_t (&m)[4])
+{
+	uint32_t r = 0, g = 0, b = 0;
+#pragma unroll
+	for (int y = 0; y < 4; ++y) {
+		r = __builtin_amdgcn_udot4(P.r[y], m[y], r, false);
+		g = __builtin_amdgcn_udot4(P.g[y], m[y], g, false);
+		b = __builtin_amdgcn_udot4(P.b[y], m[y], b, false);
+	}
+	return QSums{(int)r, (int)g, (int)b};
+}
+
+// s2tc_evaluate_colors_result_t<color_t, bigcolor_t, 1>::evaluate (:542-551)
+template <int CD>
+S2TC_D bool quad_eval(int n0, const QSums &s0, int n1, const QSums &s1, uint32_t &c0, uint32_t &c1)
+{
+	typedef QuadW<CD> W;
+	if (!n0 && !n1) return false;
+	if (n0) {
+		const Recip q((uint32_t)(n0 << 1));
+		const int r = (int)q.div((uint32_t)((s0.r << 1) + n0)) & 31;
+		const int g = (int)q.div((uint32_t)(((s0.g / W::AG) << 1) + n0)) & 63;
+		const int b = (int)q.div((uint32_t)((s0.b << 1) + n0)) & 31;
+		c0 = (uint32_t)((r << 11) | (g << 5) | b);
+	}
+	if (n1) {
+		const Recip q((uint32_t)(n1 << 1));
+		const int r = (int)q.div((uint32_t)((s1.r << 1) + n1)) & 31;
+		const int g = (int)q.div((uint32_t)(((s1.g / W::AG) << 1) + n1)) & 63;
+		const int b = (int)q.div((uint32_t)((s1.b << 1) + n1)) & 31;
+		c1 = (uint32_t)((r << 11) | (g << 5) | b);
+	}
+	return true;
+}
+
+// One s2tc_try_encode_block pass for endpoints c0, c1 (565 words).  Returns
+// the slot-ordered index bits of the used texels (1 = endpoint 1, 0 also for
+// texels that do not count), n1 and s1 = count and dot-byte sums of the
+// texels that chose endpoint 1, and the pass score sum(min(d0, d1)) when
+// SCORE (Q = sum of |t|_w^2, tot = sums over all used texels).
 template <int CD, bool SCORE>
-S2TC_D uint32_t quad_pass(const uint32_t (&U)[16], const uint32_t (&ps)[16], uint32_t use2, uint32_t c0, uint32_t c1,
-                          int Q, int ntot, uint32_t stot, uint32_t &score, int &n1, uint32_t &s1)
+S2TC_D uint32_t quad_pass(const uint32_t (&U)[16], const QuadPlanes &P, uint32_t c0, uint32_t c1, int Q, int ntot,
+                          const QSums &tot, uint32_t &score, int &n1, QSums &s1)
 {
 	typedef QuadW<CD> W;
 	const int r0 = c_r(c0), g0 = c_g(c0), b0 = c_b(c0);
@@ -97,23 +165,22 @@ S2TC_D uint32_t quad_pass(const uint32_t (&U)[16], const uint32_t (&ps)[16], uin
 	const uint32_t Wn = ((uint32_t)(-W::R * dr) & 0xffu) | (((uint32_t)(-(W::G / W::AG) * dg) & 0xffu) << 8) |
 	                    (((uint32_t)(-W::B * db) & 0xffu) << 16);
 	const int Ch = C >> 1;
-	uint32_t bits = 0, s = 0;
+	uint32_t bits = 0, m[4];
 #pragma unroll
-	for (int k = 0; k < 16; ++k) {
-		const int slot = (k & 3) * 4 + (k >> 2);
-		const uint32_t sm = (uint32_t)(__builtin_amdgcn_sdot4((int)U[k], (int)Wn, Ch, false) >> 31);
-		bits |= sm & (1u << (2 * slot));
-		s += sm & ps[k];
+	for (int y = 0; y < 4; ++y) {
+		const int v0 = __builtin_amdgcn_sdot4((int)U[y], (int)Wn, Ch, false);
+		const int v1 = __builtin_amdgcn_sdot4((int)U[4 + y], (int)Wn, Ch, false);
+		const int v2 = __builtin_amdgcn_sdot4((int)U[8 + y], (int)Wn, Ch, false);
+		const int v3 = __builtin_amdgcn_sdot4((int)U[12 + y], (int)Wn, Ch, false);
+		m[y] = sign_bytes(v0, v1, v2, v3) & P.use[y];
+		bits |= bytes_to_slots(m[y]) << (8 * y);
 	}
-	bits &= use2;
 	n1 = __builtin_popcount(bits);
-	s1 = s;
+	s1 = plane_sums(P, m);
 	if (SCORE) {
 		// sum over used texels of d0, minus sum over endpoint-1 texels of d0 - d1
-		const int Sr = (int)(stot & 1023u), Sg = (int)((stot >> 10) & 2047u), Sb = (int)(stot >> 21);
-		const int S1r = (int)(s & 1023u), S1g = (int)((s >> 10) & 2047u), S1b = (int)(s >> 21);
-		const int d0sum = Q - 2 * (W::R * Sr * r0 + W::G * Sg * g0 + W::B * Sb * b0) + ntot * B0;
-		const int gain = 2 * (W::R * dr * S1r + W::G * dg * S1g + W::B * db * S1b) - n1 * C;
+		const int d0sum = Q - 2 * (W::R * tot.r * r0 + (W::G / W::AG) * tot.g * g0 + W::B * tot.b * b0) + ntot * B0;
+		const int gain = 2 * (W::R * dr * s1.r + (W::G / W::AG) * dg * s1.g + W::B * db * s1.b) - n1 * C;
 		score = (uint32_t)(d0sum - gain);
 	}
 	return bits;
@@ -122,15 +189,12 @@ S2TC_D uint32_t quad_pass(const uint32_t (&U)[16], const uint32_t (&ps)[16], uin
 // colour refinement NEVER/ALWAYS/LOOP (:767-860) with the quadratic pass;
 // trans3 holds index 3 in the slots of DXT1 transparent texels.
 template <int CD, bool TRANS>
-S2TC_D uint32_t quad_color_refine(const uint32_t (&U)[16], const uint32_t (&ps)[16], uint32_t use2, uint32_t trans3,
-                                  int Q, int refine, uint32_t &c0, uint32_t &c1)
+S2TC_D uint32_t quad_color_refine(const uint32_t (&U)[16], const QuadPlanes &P, int ntot, uint32_t trans3, int Q,
+                                  int refine, uint32_t &c0, uint32_t &c1)
 {
-	const int ntot = __builtin_popcount(use2);
-	uint32_t stot = 0;
-#pragma unroll
-	for (int k = 0; k < 16; ++k) stot += ps[k];
-	uint32_t score, s1;
+	uint32_t score;
 	int n1;
+	QSums s1;
 	uint32_t bits;
 	if (refine == REFINE_NEVER) {
 		if (TRANS ? c1 < c0 : c0 < c1) {
@@ -138,23 +202,27 @@ S2TC_D uint32_t quad_color_refine(const uint32_t (&U)[16], const uint32_t (&ps)[
 			c0 = c1;
 			c1 = t;
 		}
-		return quad_pass<CD, false>(U, ps, use2, c0, c1, Q, ntot, stot, score, n1, s1) | trans3;
+		const QSums none{0, 0, 0};
+		return quad_pass<CD, false>(U, P, c0, c1, Q, ntot, none, score, n1, s1) | trans3;
 	}
+	const QSums tot = plane_sums(P, P.use);
 	if (refine == REFINE_LOOP) {
 		bits = 0;
 		uint32_t s = 0x7fffffffu, n0v = c0, n1v = c1;
 		for (int it = 0; it < S2TC_LOOP_CAP; ++it) {
-			const uint32_t b2 = quad_pass<CD, true>(U, ps, use2, n0v, n1v, Q, ntot, stot, score, n1, s1);
+			const uint32_t b2 = quad_pass<CD, true>(U, P, n0v, n1v, Q, ntot, tot, score, n1, s1);
 			if (!(score < s)) break;
 			bits = b2;
 			s = score;
 			c0 = n0v;
 			c1 = n1v;
-			if (!eval_color(ntot - n1, stot - s1, n1, s1, n0v, n1v)) break;
+			const QSums s0{tot.r - s1.r, tot.g - s1.g, tot.b - s1.b};
+			if (!quad_eval<CD>(ntot - n1, s0, n1, s1, n0v, n1v)) break;
 		}
 	} else {
-		bits = quad_pass<CD, false>(U, ps, use2, c0, c1, Q, ntot, stot, score, n1, s1);
-		eval_color(ntot - n1, stot - s1, n1, s1, c0, c1);
+		bits = quad_pass<CD, false>(U, P, c0, c1, Q, ntot, tot, score, n1, s1);
+		const QSums s0{tot.r - s1.r, tot.g - s1.g, tot.b - s1.b};
+		quad_eval<CD>(ntot - n1, s0, n1, s1, c0, c1);
 	}
 	bits |= trans3;
 	color_fix_order<TRANS>(bits, c0, c1);
@@ -164,31 +232,33 @@ S2TC_D uint32_t quad_color_refine(const uint32_t (&U)[16], const uint32_t (&ps)[
 // The whole quick-mode block for a quadratic distance.  px[k]: source word
 // of texel x = k >> 2, y = k & 3 (0 outside w x h); blk: the source at the
 // block's top-left texel, rows `width` words apart.  FULL: the block is 4x4.
-template <int CD, int DXT, bool FULL, bool RAW>
+// OPAQUE (DXT1): every texel is known to be opaque.
+template <int CD, int DXT, bool FULL, bool RAW, bool OPAQUE>
 S2TC_D BlockOut encode_block_quad(const uint32_t (&px)[16], const uint32_t *__restrict__ blk, int width, int w, int h,
                                   int refine)
 {
 	typedef QuadTexel<CD, RAW> T;
-	uint32_t U[16], ps[16];
-	uint32_t use2 = 0, valid2 = 0;
+	constexpr bool ALL = FULL && (DXT != DXT1 || OPAQUE);  // every texel counts
+	uint32_t U[16];
+	QuadPlanes P;
+	uint32_t usem[16];
 	uint32_t kmax = 0u, kmin = 0xffffffffu;
 	int Q = 0;
 	int a0 = 0, a1 = 0;
 	if (DXT == DXT5) a0 = a1 = T::alpha(px[0], DXT5);  // (:894-898)
 #pragma unroll
 	for (int k = 0; k < 16; ++k) {
-		const int x = k >> 2, y = k & 3, slot = y * 4 + x;
+		const int x = k >> 2, y = k & 3;
 		const bool valid = FULL || (x < w && y < h);
 		const T t(px[k]);
-		uint32_t usem = valid ? ~0u : 0u;
-		if (DXT == DXT1) usem &= T::opaque(px[k]);
+		uint32_t um = valid ? ~0u : 0u;
+		if (DXT == DXT1 && !OPAQUE) um &= T::opaque(px[k]);
+		if (ALL) um = ~0u;
+		usem[k] = um;
 		U[k] = t.u;
-		ps[k] = t.ps & usem;
-		use2 |= usem & (1u << (2 * slot));
-		valid2 |= valid ? 1u << (2 * slot) : 0u;
 		// darkest / brightest by distance to black, first occurrence (:900-922)
-		const uint32_t dx = (uint32_t)t.d & usem;
-		const uint32_t dn = (usem & (uint32_t)t.d) | (~usem & 0x0fffffffu);
+		const uint32_t dx = ALL ? (uint32_t)t.d : (uint32_t)t.d & um;
+		const uint32_t dn = ALL ? (uint32_t)t.d : ((um & (uint32_t)t.d) | (~um & 0x0fffffffu));
 		Q += (int)dx;
 		kmax = max(kmax, (dx << 4) | (uint32_t)(15 - k));
 		kmin = min(kmin, (dn << 4) | (uint32_t)k);
@@ -199,6 +269,25 @@ S2TC_D BlockOut encode_block_quad(const uint32_t (&px)[16], const uint32_t *__re
 			a0 = (ta && a < a0) ? a : a0;
 		}
 	}
+	quad_planes(U, P);
+	uint32_t use2 = 0x55555555u, valid2 = 0x55555555u;
+	if (ALL) {
+#pragma unroll
+		for (int y = 0; y < 4; ++y) P.use[y] = 0x01010101u;
+	} else {
+		use2 = 0;
+#pragma unroll
+		for (int y = 0; y < 4; ++y) {
+			P.use[y] = sign_bytes((int)usem[y], (int)usem[4 + y], (int)usem[8 + y], (int)usem[12 + y]);
+			use2 |= bytes_to_slots(P.use[y]) << (8 * y);
+		}
+		if (!FULL) {
+			valid2 = 0;
+#pragma unroll
+			for (int k = 0; k < 16; ++k) valid2 |= ((k >> 2) < w && (k & 3) < h ? 1u : 0u) << (2 * ((k & 3) * 4 + (k >> 2)));
+		}
+	}
+	const int ntot = __builtin_popcount(use2);
 	// the winners, read back from the source (texel 0 stands in when there
 	// is none; its value is then not used); no used texel leaves the dummies
 	// (31,63,31) and (0,0,0) of :886-891
@@ -212,12 +301,12 @@ S2TC_D BlockOut encode_block_quad(const uint32_t (&px)[16], const uint32_t *__re
 	BlockOut o;
 	if (DXT == DXT1) {
 		const uint32_t trans3 = (valid2 & ~use2) * 3u;
-		const uint32_t bits = quad_color_refine<CD, true>(U, ps, use2, trans3, Q, refine, c0, c1);
+		const uint32_t bits = quad_color_refine<CD, true>(U, P, ntot, trans3, Q, refine, c0, c1);
 		o.w[0] = c0 | (c1 << 16);
 		o.w[1] = bits;
 		o.w[2] = o.w[3] = 0;
 	} else if (DXT == DXT3) {
-		const uint32_t bits = quad_color_refine<CD, false>(U, ps, use2, 0u, Q, refine, c0, c1);
+		const uint32_t bits = quad_color_refine<CD, false>(U, P, ntot, 0u, Q, refine, c0, c1);
 		uint64_t ab = 0;
 #pragma unroll
 		for (int k = 0; k < 16; ++k) {
@@ -231,7 +320,7 @@ S2TC_D BlockOut encode_block_quad(const uint32_t (&px)[16], const uint32_t *__re
 		o.w[3] = bits;
 	} else {
 		a1 = fix_equal_a(a0, a1);
-		const uint32_t bits = quad_color_refine<CD, false>(U, ps, use2, 0u, Q, refine, c0, c1);
+		const uint32_t bits = quad_color_refine<CD, false>(U, P, ntot, 0u, Q, refine, c0, c1);
 		// the alpha byte is the same in RAW and quantized DXT5 words
 		uint32_t valid = 0xffffu;
 		if (!FULL) {
@@ -249,12 +338,22 @@ S2TC_D BlockOut encode_block_quad(const uint32_t (&px)[16], const uint32_t *__re
 	return o;
 }
 
+// Full blocks take the 4x4 form; DXT1 waves whose blocks are all full and
+// opaque (a wave-uniform vote) skip the transparency masks.
 template <int CD, int DXT, bool RAW>
 S2TC_D BlockOut encode_block_quad_any(const uint32_t (&px)[16], const uint32_t *__restrict__ blk, int width, int w,
                                       int h, int refine)
 {
-	if (w == 4 && h == 4) return encode_block_quad<CD, DXT, true, RAW>(px, blk, width, 4, 4, refine);
-	return encode_block_quad<CD, DXT, false, RAW>(px, blk, width, w, h, refine);
+	typedef QuadTexel<CD, RAW> T;
+	const bool full = w == 4 && h == 4;
+	if (DXT == DXT1) {
+		uint32_t op = full ? ~0u : 0u;
+#pragma unroll
+		for (int k = 0; k < 16; ++k) op &= T::opaque(px[k]);
+		if (__all(op != 0u)) return encode_block_quad<CD, DXT, true, RAW, true>(px, blk, width, 4, 4, refine);
+	}
+	if (full) return encode_block_quad<CD, DXT, true, RAW, false>(px, blk, width, 4, 4, refine);
+	return encode_block_quad<CD, DXT, false, RAW, false>(px, blk, width, w, h, refine);
 }
 
 }  // namespace s2tc

@@ -20,6 +20,7 @@ struct TileJobs {
 
 struct KernelTable {
 	const void *fast[3][2];       // [dxt][raw]: k_fast<CD, dxt, raw>, null for NORMALMAP
+	const void *fast_int[3][2][3];  // [dxt][raw][refine]: k_fast_interior (full blocks, 16-byte rows), null for NORMALMAP
 	const void *normal[2];        // [raw]: k_normal<CD, raw>
 	int normal_smem_per_cand;     // LDS bytes per candidate slot for a 4-block workgroup
 	const void *fast_batch[3];    // [dxt]: persistent quick-mode kernel over a job list

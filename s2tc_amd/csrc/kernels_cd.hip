@@ -27,6 +27,15 @@ S2TC_D uint32_t quant_none_rt(uint32_t p, int dxt)
 	return dxt == DXT1 ? quant_none<DXT1>(p) : dxt == DXT3 ? quant_none<DXT3>(p) : quant_none<DXT5>(p);
 }
 
+// smallest of the 16 source words: its top byte is the smallest alpha
+S2TC_D uint32_t tx_min16(const uint32_t (&tx)[16])
+{
+	uint32_t m = tx[0];
+#pragma unroll
+	for (int k = 1; k < 16; ++k) m = min(m, tx[k]);
+	return m;
+}
+
 // Load the 4x4 block (bx, by) into tx[x*4+y]; texels outside w x h are 0.
 template <int DXT, bool RAW>
 S2TC_D void load_block(const uint32_t *__restrict__ src, int width, int bx, int by, int w, int h, bool vec,
@@ -65,16 +74,24 @@ template <int DXT> S2TC_D void store_block(uint8_t *dst, size_t b, const BlockOu
 		*(uint4 *)(dst + b * 16) = make_uint4(o.w[0], o.w[1], o.w[2], o.w[3]);
 }
 
-// Quick mode over one image: one lane per block, blocks in raster order.
-template <int CD, int DXT, bool RAW>
-__global__ __launch_bounds__(256) void k_fast(const uint32_t *__restrict__ src, int width, int height,
-                                              uint8_t *__restrict__ dst, int refine, int vec)
-{
-	const int bw = (width + 3) >> 2, bh = (height + 3) >> 2;
-	const int b = (int)(blockIdx.x * 256 + threadIdx.x);
-	if (b >= bw * bh) return;
-	const int by = b / bw, bx = b - by * bw;
+// Single-image kernels run 64 x 4 lanes per workgroup on a 2-D grid: lane
+// (x, y) takes block (64 blockIdx.x + x, by0 + 4 blockIdx.y + y), so each row
+// of 64 lanes reads 64 consecutive blocks (1 KiB per texel row) and no lane
+// divides by the block-row width.
+#define S2TC_IMG_BLOCK_COORDS                                                        \
+	const int bw = (width + 3) >> 2, bh = (height + 3) >> 2;                         \
+	const int bx = (int)(blockIdx.x * 64 + threadIdx.x);                             \
+	const int by = by0 + (int)(blockIdx.y * 4 + threadIdx.y);                        \
+	if (bx >= bw || by >= bh) return;                                                \
+	const size_t b = (size_t)by * (size_t)bw + (size_t)bx;                          \
 	const int w = min(4, width - 4 * bx), h = min(4, height - 4 * by);
+
+// Quick mode over one image: one lane per block.
+template <int CD, int DXT, bool RAW>
+__global__ __launch_bounds__(256, 4) void k_fast(const uint32_t *__restrict__ src, int width, int height,
+                                                 uint8_t *__restrict__ dst, int refine, int vec, int by0)
+{
+	S2TC_IMG_BLOCK_COORDS
 	uint32_t tx[16];
 	BlockOut o;
 	if constexpr (QuadW<CD>::value) {
@@ -85,23 +102,58 @@ __global__ __launch_bounds__(256) void k_fast(const uint32_t *__restrict__ src, 
 		load_block<DXT, RAW>(src, width, bx, by, w, h, vec != 0, tx);
 		o = encode_block_fast<CD, DXT>(tx, w, h, refine);
 	}
-	store_block<DXT>(dst, (size_t)b, o);
+	store_block<DXT>(dst, b, o);
+}
+
+// Quick mode over the full 4x4 blocks of an image whose rows are 16-byte
+// aligned (width % 4 == 0): blocks (bx, by) with bx < nbx, by0 <= by < nby.
+// Without partial-block code the kernel needs far fewer registers, so more
+// waves hide the loads; the edge blocks go to k_fast.
+template <int CD, int DXT, bool RAW, int REFINE>
+__global__ __launch_bounds__(256) void k_fast_interior(const uint32_t *__restrict__ src, int width, int nbx, int nby,
+                                                       uint8_t *__restrict__ dst, int by0)
+{
+	const int refine = REFINE;
+	const int bx = (int)(blockIdx.x * 64 + threadIdx.x);
+	const int by = by0 + (int)(blockIdx.y * 4 + threadIdx.y);
+	if (bx >= nbx || by >= nby) return;
+	const size_t b = (size_t)by * (size_t)((width + 3) >> 2) + (size_t)bx;
+	const uint32_t *blk = src + (size_t)(4 * by) * (size_t)width + 4 * bx;
+	uint32_t tx[16];
+#pragma unroll
+	for (int y = 0; y < 4; ++y) {
+		const uint4 q = *(const uint4 *)(blk + (size_t)y * (size_t)width);
+		tx[y] = q.x;
+		tx[4 + y] = q.y;
+		tx[8 + y] = q.z;
+		tx[12 + y] = q.w;
+	}
+	BlockOut o;
+	if constexpr (QuadW<CD>::value) {
+		if (DXT == DXT1 && __all((int)(RAW ? tx_min16(tx) >> 31 : tx_min16(tx) >> 24) != 0))
+			o = encode_block_quad<CD, DXT, true, RAW, true>(tx, blk, width, 4, 4, refine);
+		else
+			o = encode_block_quad<CD, DXT, true, RAW, false>(tx, blk, width, 4, 4, refine);
+	} else {
+		if (RAW) {
+#pragma unroll
+			for (int k = 0; k < 16; ++k) tx[k] = quant_none<DXT>(tx[k]);
+		}
+		o = encode_block_fast_t<CD, DXT, true>(tx, 4, 4, refine);
+	}
+	store_block<DXT>(dst, b, o);
 }
 
 // MODE_NORMAL without random colours over one image: one lane per block.
 template <int CD, int DXT, bool RAW>
 __global__ __launch_bounds__(256) void k_all16(const uint32_t *__restrict__ src, int width, int height,
-                                               uint8_t *__restrict__ dst, int refine, int vec)
+                                               uint8_t *__restrict__ dst, int refine, int vec, int by0)
 {
-	const int bw = (width + 3) >> 2, bh = (height + 3) >> 2;
-	const int b = (int)(blockIdx.x * 256 + threadIdx.x);
-	if (b >= bw * bh) return;
-	const int by = b / bw, bx = b - by * bw;
-	const int w = min(4, width - 4 * bx), h = min(4, height - 4 * by);
+	S2TC_IMG_BLOCK_COORDS
 	uint32_t tx[16];
 	load_block<DXT, RAW>(src, width, bx, by, w, h, vec != 0, tx);
 	const BlockOut o = encode_block_all16<CD, DXT>(tx, w, h, refine);
-	store_block<DXT>(dst, (size_t)b, o);
+	store_block<DXT>(dst, b, o);
 }
 
 // MODE_NORMAL over one image: 16 lanes per block, 4 blocks per 64-lane workgroup.
@@ -277,8 +329,18 @@ extern "C" void S2TC_CAT(s2tc_kernels_cd, S2TC_CD)(KernelTable *kt)
 	kt->fast[DXT1][1] = (const void *)k_fast<S2TC_CD, DXT1, true>;
 	kt->fast[DXT3][1] = (const void *)k_fast<S2TC_CD, DXT3, true>;
 	kt->fast[DXT5][1] = (const void *)k_fast<S2TC_CD, DXT5, true>;
+#define S2TC_INT3(D, R)                                                                      \
+	kt->fast_int[D][R][0] = (const void *)k_fast_interior<S2TC_CD, D, R, REFINE_NEVER>;  \
+	kt->fast_int[D][R][1] = (const void *)k_fast_interior<S2TC_CD, D, R, REFINE_ALWAYS>; \
+	kt->fast_int[D][R][2] = (const void *)k_fast_interior<S2TC_CD, D, R, REFINE_LOOP>;
+	S2TC_INT3(DXT1, false) S2TC_INT3(DXT3, false) S2TC_INT3(DXT5, false)
+	S2TC_INT3(DXT1, true) S2TC_INT3(DXT3, true) S2TC_INT3(DXT5, true)
+#undef S2TC_INT3
 #else
-	for (int d = 0; d < 3; ++d) kt->fast[d][0] = kt->fast[d][1] = nullptr;
+	for (int d = 0; d < 3; ++d) {
+		kt->fast[d][0] = kt->fast[d][1] = nullptr;
+		for (int r = 0; r < 3; ++r) kt->fast_int[d][0][r] = kt->fast_int[d][1][r] = nullptr;
+	}
 #endif
 #if S2TC_CD != 7
 	kt->fast_batch[DXT1] = (const void *)k_fast_batch<S2TC_CD, DXT1>;

@@ -244,9 +244,9 @@ struct Recip {
 		uint32_t q = (uint32_t)(262144.0f * __builtin_amdgcn_rcpf((float)d));
 		const int r = 262144 - (int)__umul24(q, d);  // exact correction of the estimate
 		q = r < 0 ? q - 1u : ((uint32_t)r >= d ? q + 1u : q);
-		m = q + 1u;
+		m = (q + 1u) & 0x3ffffu;  // <= 131073; the mask lets the product use the 24-bit multiplier
 	}
-	S2TC_D uint32_t div(uint32_t x) const { return __umul24(x, m) >> 18; }
+	S2TC_D uint32_t div(uint32_t x) const { return ((x & 0x1fffu) * m) >> 18; }
 };
 // ((S << 1) + n) / (n << 1): round-half-up mean (:547-549); s <= 255 n, n <= 16
 S2TC_D int rmean(int s, int n) { return (int)Recip((uint32_t)(n << 1)).div((uint32_t)((s << 1) + n)); }

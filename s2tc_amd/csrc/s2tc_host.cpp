@@ -122,18 +122,35 @@ int launch_encode(const EncodeArgs &a, hipStream_t stream)
 	const void *src = a.src;
 	void *dst = a.dst;
 	int width = a.width, height = a.height;
-	if (is_fast(cd, a.nrandom)) {
-		if (!kt.fast[dxt][raw]) return set_error(S2TC_E_ARG, "no quick-mode kernel for distance %d", cd);
+	if (is_fast(cd, a.nrandom) || a.nrandom <= 0) {
+		// one lane per block: quick mode, or only the block's colours
+		const void *k = is_fast(cd, a.nrandom) ? kt.fast[dxt][raw] : kt.all16[dxt][raw];
+		if (!k) return set_error(S2TC_E_ARG, "no one-lane kernel for distance %d", cd);
 		int vec = ((width & 3) == 0 && (((uintptr_t)src) & 15) == 0) ? 1 : 0;
-		void *args[] = {(void *)&src, (void *)&width, (void *)&height, (void *)&dst, (void *)&refine, (void *)&vec};
-		dim3 grid((unsigned)((nb + 255) / 256)), block(256);
-		return hip_check(hipLaunchKernel(kt.fast[dxt][raw], grid, block, args, 0, stream), "launch k_fast");
-	}
-	if (a.nrandom <= 0) {  // only the block's colours: one lane per block
-		int vec = ((width & 3) == 0 && (((uintptr_t)src) & 15) == 0) ? 1 : 0;
-		void *args[] = {(void *)&src, (void *)&width, (void *)&height, (void *)&dst, (void *)&refine, (void *)&vec};
-		dim3 grid((unsigned)((nb + 255) / 256)), block(256);
-		return hip_check(hipLaunchKernel(kt.all16[dxt][raw], grid, block, args, 0, stream), "launch k_all16");
+		const int bw = (width + 3) / 4, bh = (height + 3) / 4;
+		const int rows_per_launch = 4 * 65535;  // grid.y limit of 65535 workgroups of 4 block rows
+		int first_general_row = 0;
+		const void *ki = is_fast(cd, a.nrandom) ? kt.fast_int[dxt][raw][refine] : nullptr;
+		if (vec && ki) {
+			// full blocks of 16-byte rows first, then the partial bottom block row
+			int nbx = width / 4, nby = height / 4;
+			for (int by0 = 0; by0 < nby; by0 += rows_per_launch) {
+				const int rows = nby - by0 < rows_per_launch ? nby - by0 : rows_per_launch;
+				void *args[] = {(void *)&src, (void *)&width, (void *)&nbx, (void *)&nby, (void *)&dst, (void *)&by0};
+				dim3 grid((unsigned)((nbx + 63) / 64), (unsigned)((rows + 3) / 4)), block(64, 4);
+				if (int r = hip_check(hipLaunchKernel(ki, grid, block, args, 0, stream), "launch k_fast_interior"))
+					return r;
+			}
+			first_general_row = nby;
+		}
+		for (int by0 = first_general_row; by0 < bh; by0 += rows_per_launch) {
+			const int rows = bh - by0 < rows_per_launch ? bh - by0 : rows_per_launch;
+			void *args[] = {(void *)&src, (void *)&width, (void *)&height, (void *)&dst,
+			                (void *)&refine, (void *)&vec, (void *)&by0};
+			dim3 grid((unsigned)((bw + 63) / 64), (unsigned)((rows + 3) / 4)), block(64, 4);
+			if (int r = hip_check(hipLaunchKernel(k, grid, block, args, 0, stream), "launch one-lane encoder")) return r;
+		}
+		return 0;
 	}
 	int nrandom = a.nrandom;
 	int pbw = normal_pbw(nrandom);
