@@ -218,14 +218,18 @@ S2TC_D uint64_t alpha_try_thr(const uint32_t (&tx)[16], uint32_t valid, int a0, 
 {
 	const int zh = min(a0 == 0 ? 1023 : a0, a1 == 0 ? 1023 : a1) >> 1;                // 0 wins  <=>  a <= zh
 	const int fc = (max(a0 == 255 ? 0 : 255 + a0, a1 == 255 ? 0 : 255 + a1) + 1) >> 1;  // 255 wins <=> a >= fc
-	const int D2 = (a1 - a0) * 2, SD = (a0 + a1) * (a1 - a0);                           // e1 nearer <=> a D2 > SD
+	// e1 nearer <=> (e1 - e0)(2a - e0 - e1) > 0: a > floor(S / 2) when e1 > e0,
+	// a < ceil(S / 2) (255 - a > 255 - ceil(S / 2)) when e1 < e0, never when equal
+	const int S = a0 + a1;
+	const int flip = a1 < a0 ? 255 : 0;
+	const int tb = a1 > a0 ? S >> 1 : a1 < a0 ? 255 - ((S + 1) >> 1) : 255;
 	uint32_t lo = 0, hi = 0, cacc = 0, c1 = 0;
 #pragma unroll
 	for (int k = 0; k < 16; ++k) {
 		const int x = k >> 2, y = k & 3, slot = y * 4 + x;
 		const bool v = FULL || ((valid >> k) & 1u);
 		const int a = texel_a(tx[k]);
-		const bool z = a <= zh, full = !z && a >= fc, b = a * D2 > SD;
+		const bool z = a <= zh, full = !z && a >= fc, b = (a ^ flip) > tb;
 		const bool acc = v && !z && !full;
 		const uint32_t idx = v ? (z ? 6u : full ? 7u : (b ? 1u : 0u)) : 0u;
 		if (3 * slot + 2 < 32) {
@@ -377,11 +381,27 @@ S2TC_D BlockOut encode_block_fast_t(const uint32_t (&tx)[16], int w, int h, int 
 	return o;
 }
 
+// An empty asm that claims to rewrite the words: work derived from them is
+// then computed inside each branch instead of being hoisted above it, which
+// would keep it live (in registers) across both paths.
+S2TC_D void opaque_to_compiler(uint32_t (&tx)[16])
+{
+#pragma unroll
+	for (int k = 0; k < 16; ++k) asm volatile("" : "+v"(tx[k]));
+}
+
 template <int CD, int DXT>
 S2TC_D BlockOut encode_block_fast(const uint32_t (&tx)[16], int w, int h, int refine)
 {
-	if (w == 4 && h == 4) return encode_block_fast_t<CD, DXT, true>(tx, 4, 4, refine);
-	return encode_block_fast_t<CD, DXT, false>(tx, w, h, refine);
+	uint32_t t[16];
+#pragma unroll
+	for (int k = 0; k < 16; ++k) t[k] = tx[k];
+	if (w == 4 && h == 4) {
+		opaque_to_compiler(t);
+		return encode_block_fast_t<CD, DXT, true>(t, 4, 4, refine);
+	}
+	opaque_to_compiler(t);
+	return encode_block_fast_t<CD, DXT, false>(t, w, h, refine);
 }
 
 }  // namespace s2tc

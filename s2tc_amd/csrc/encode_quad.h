@@ -278,7 +278,11 @@ S2TC_D BlockOut encode_block_quad(const uint32_t (&px)[16], const uint32_t *__re
 		use2 = 0;
 #pragma unroll
 		for (int y = 0; y < 4; ++y) {
-			P.use[y] = sign_bytes((int)usem[y], (int)usem[4 + y], (int)usem[8 + y], (int)usem[12 + y]);
+			// full RAW blocks: a texel counts iff alpha >= 128, the sign bit of its word
+			if (FULL && RAW)
+				P.use[y] = sign_bytes((int)px[y], (int)px[4 + y], (int)px[8 + y], (int)px[12 + y]);
+			else
+				P.use[y] = sign_bytes((int)usem[y], (int)usem[4 + y], (int)usem[8 + y], (int)usem[12 + y]);
 			use2 |= bytes_to_slots(P.use[y]) << (8 * y);
 		}
 		if (!FULL) {
@@ -346,14 +350,24 @@ S2TC_D BlockOut encode_block_quad_any(const uint32_t (&px)[16], const uint32_t *
 {
 	typedef QuadTexel<CD, RAW> T;
 	const bool full = w == 4 && h == 4;
+	uint32_t t[16];
+#pragma unroll
+	for (int k = 0; k < 16; ++k) t[k] = px[k];
 	if (DXT == DXT1) {
 		uint32_t op = full ? ~0u : 0u;
 #pragma unroll
-		for (int k = 0; k < 16; ++k) op &= T::opaque(px[k]);
-		if (__all(op != 0u)) return encode_block_quad<CD, DXT, true, RAW, true>(px, blk, width, 4, 4, refine);
+		for (int k = 0; k < 16; ++k) op &= T::opaque(t[k]);
+		if (__all(op != 0u)) {
+			opaque_to_compiler(t);
+			return encode_block_quad<CD, DXT, true, RAW, true>(t, blk, width, 4, 4, refine);
+		}
 	}
-	if (full) return encode_block_quad<CD, DXT, true, RAW, false>(px, blk, width, 4, 4, refine);
-	return encode_block_quad<CD, DXT, false, RAW, false>(px, blk, width, w, h, refine);
+	if (full) {
+		opaque_to_compiler(t);
+		return encode_block_quad<CD, DXT, true, RAW, false>(t, blk, width, 4, 4, refine);
+	}
+	opaque_to_compiler(t);
+	return encode_block_quad<CD, DXT, false, RAW, false>(t, blk, width, w, h, refine);
 }
 
 }  // namespace s2tc

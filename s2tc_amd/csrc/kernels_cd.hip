@@ -130,10 +130,13 @@ __global__ __launch_bounds__(256) void k_fast_interior(const uint32_t *__restric
 	}
 	BlockOut o;
 	if constexpr (QuadW<CD>::value) {
-		if (DXT == DXT1 && __all((int)(RAW ? tx_min16(tx) >> 31 : tx_min16(tx) >> 24) != 0))
+		if (DXT == DXT1 && __all((int)(RAW ? tx_min16(tx) >> 31 : tx_min16(tx) >> 24) != 0)) {
+			opaque_to_compiler(tx);
 			o = encode_block_quad<CD, DXT, true, RAW, true>(tx, blk, width, 4, 4, refine);
-		else
+		} else {
+			opaque_to_compiler(tx);
 			o = encode_block_quad<CD, DXT, true, RAW, false>(tx, blk, width, 4, 4, refine);
+		}
 	} else {
 		if (RAW) {
 #pragma unroll
@@ -207,7 +210,7 @@ S2TC_D int find_job(const JobDesc *__restrict__ jobs, int lo, int hi, long long 
 	return lo;
 }
 
-template <int CD, int DXT>
+template <int CD, int DXT, int REFINE>
 __global__ __launch_bounds__(256) void k_fast_batch(const JobDesc *__restrict__ jobs, const TileJobs *__restrict__ tile_jobs,
                                                     long long nblocks, int ntiles)
 {
@@ -226,12 +229,12 @@ __global__ __launch_bounds__(256) void k_fast_batch(const JobDesc *__restrict__ 
 		if constexpr (QuadW<CD>::value) {
 			load_block<DXT, false>(j.src, j.width, bx, by, w, h, vec, tx);
 			const uint32_t *blk = j.src + (size_t)(4 * by) * (size_t)j.width + 4 * bx;
-			if (j.raw) o = encode_block_quad_any<CD, DXT, true>(tx, blk, j.width, w, h, j.refine);
-			else o = encode_block_quad_any<CD, DXT, false>(tx, blk, j.width, w, h, j.refine);
+			if (j.raw) o = encode_block_quad_any<CD, DXT, true>(tx, blk, j.width, w, h, REFINE);
+			else o = encode_block_quad_any<CD, DXT, false>(tx, blk, j.width, w, h, REFINE);
 		} else {
 			if (j.raw) load_block<DXT, true>(j.src, j.width, bx, by, w, h, vec, tx);
 			else load_block<DXT, false>(j.src, j.width, bx, by, w, h, vec, tx);
-			o = encode_block_fast<CD, DXT>(tx, w, h, j.refine);
+			o = encode_block_fast<CD, DXT>(tx, w, h, REFINE);
 		}
 		store_block<DXT>(j.dst, (size_t)b, o);
 	}
@@ -343,11 +346,14 @@ extern "C" void S2TC_CAT(s2tc_kernels_cd, S2TC_CD)(KernelTable *kt)
 	}
 #endif
 #if S2TC_CD != 7
-	kt->fast_batch[DXT1] = (const void *)k_fast_batch<S2TC_CD, DXT1>;
-	kt->fast_batch[DXT3] = (const void *)k_fast_batch<S2TC_CD, DXT3>;
-	kt->fast_batch[DXT5] = (const void *)k_fast_batch<S2TC_CD, DXT5>;
+#define S2TC_FB3(D)                                                                \
+	kt->fast_batch[D][0] = (const void *)k_fast_batch<S2TC_CD, D, REFINE_NEVER>;  \
+	kt->fast_batch[D][1] = (const void *)k_fast_batch<S2TC_CD, D, REFINE_ALWAYS>; \
+	kt->fast_batch[D][2] = (const void *)k_fast_batch<S2TC_CD, D, REFINE_LOOP>;
+	S2TC_FB3(DXT1) S2TC_FB3(DXT3) S2TC_FB3(DXT5)
+#undef S2TC_FB3
 #else
-	for (int d = 0; d < 3; ++d) kt->fast_batch[d] = nullptr;
+	for (int d = 0; d < 3; ++d) kt->fast_batch[d][0] = kt->fast_batch[d][1] = kt->fast_batch[d][2] = nullptr;
 #endif
 	kt->normal_batch = (const void *)k_normal_batch<S2TC_CD>;
 	kt->all16[DXT1][0] = (const void *)k_all16<S2TC_CD, DXT1, false>;
