@@ -26,7 +26,7 @@ struct KernelTable {
 	const void *fast_batch[3][3]; // [dxt][refine]: persistent quick-mode kernel over a job list
 	const void *normal_batch;     // persistent MODE_NORMAL kernel over a job list
 	const void *all16[3][2];      // [dxt][raw]: MODE_NORMAL without random colours, one lane per block
-	const void *all16_batch[3];   // [dxt]: its persistent batch form
+	const void *all16_batch[3][3];  // [dxt][refine]: its persistent batch form
 	const void *init;             // per-device table set-up kernel (256 x 256 threads, one int argument), or null
 };
 

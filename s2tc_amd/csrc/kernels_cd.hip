@@ -240,7 +240,7 @@ __global__ __launch_bounds__(256) void k_fast_batch(const JobDesc *__restrict__ 
 	}
 }
 
-template <int CD, int DXT>
+template <int CD, int DXT, int REFINE>
 __global__ __launch_bounds__(256) void k_all16_batch(const JobDesc *__restrict__ jobs,
                                                      const TileJobs *__restrict__ tile_jobs, long long nblocks,
                                                      int ntiles)
@@ -258,7 +258,7 @@ __global__ __launch_bounds__(256) void k_all16_batch(const JobDesc *__restrict__
 		uint32_t tx[16];
 		if (j.raw) load_block<DXT, true>(j.src, j.width, bx, by, w, h, vec, tx);
 		else load_block<DXT, false>(j.src, j.width, bx, by, w, h, vec, tx);
-		const BlockOut o = encode_block_all16<CD, DXT>(tx, w, h, j.refine);
+		const BlockOut o = encode_block_all16<CD, DXT>(tx, w, h, REFINE);
 		store_block<DXT>(j.dst, (size_t)b, o);
 	}
 }
@@ -362,9 +362,12 @@ extern "C" void S2TC_CAT(s2tc_kernels_cd, S2TC_CD)(KernelTable *kt)
 	kt->all16[DXT1][1] = (const void *)k_all16<S2TC_CD, DXT1, true>;
 	kt->all16[DXT3][1] = (const void *)k_all16<S2TC_CD, DXT3, true>;
 	kt->all16[DXT5][1] = (const void *)k_all16<S2TC_CD, DXT5, true>;
-	kt->all16_batch[DXT1] = (const void *)k_all16_batch<S2TC_CD, DXT1>;
-	kt->all16_batch[DXT3] = (const void *)k_all16_batch<S2TC_CD, DXT3>;
-	kt->all16_batch[DXT5] = (const void *)k_all16_batch<S2TC_CD, DXT5>;
+#define S2TC_AB3(D)                                                                  \
+	kt->all16_batch[D][0] = (const void *)k_all16_batch<S2TC_CD, D, REFINE_NEVER>;  \
+	kt->all16_batch[D][1] = (const void *)k_all16_batch<S2TC_CD, D, REFINE_ALWAYS>; \
+	kt->all16_batch[D][2] = (const void *)k_all16_batch<S2TC_CD, D, REFINE_LOOP>;
+	S2TC_AB3(DXT1) S2TC_AB3(DXT3) S2TC_AB3(DXT5)
+#undef S2TC_AB3
 	kt->normal[0] = (const void *)k_normal<S2TC_CD, false>;
 	kt->normal[1] = (const void *)k_normal<S2TC_CD, true>;
 #if defined(S2TC_NM_TABLE)

@@ -24,8 +24,9 @@ int plan_classes(const s2tc_amd_job *jobs, int njobs, std::vector<ClassPlan> &cl
 {
 	classes.clear();
 	// class slots: 0..71 quick ((cd*3 + dxt)*3 + refine), 72..79 MODE_NORMAL
-	// with random colours (cd), 80..103 MODE_NORMAL without (80 + cd*3 + dxt)
-	int slot[104];
+	// with random colours (cd), 80..151 MODE_NORMAL without
+	// (80 + (cd*3 + dxt)*3 + refine)
+	int slot[152];
 	for (int &v : slot) v = -1;
 	for (int i = 0; i < njobs; ++i) {
 		const s2tc_amd_job &j = jobs[i];
@@ -39,7 +40,7 @@ int plan_classes(const s2tc_amd_job *jobs, int njobs, std::vector<ClassPlan> &cl
 			return set_error(S2TC_E_ARG, "job %d: bad source kind", i);
 		const bool fast = is_fast(cd, j.nrandom);
 		const bool all16 = !fast && j.nrandom <= 0;
-		const int key = fast ? (cd * 3 + dxt) * 3 + refine : all16 ? 80 + cd * 3 + dxt : 72 + cd;
+		const int key = fast ? (cd * 3 + dxt) * 3 + refine : all16 ? 80 + (cd * 3 + dxt) * 3 + refine : 72 + cd;
 		if (slot[key] < 0) {
 			slot[key] = (int)classes.size();
 			classes.emplace_back();
@@ -48,7 +49,7 @@ int plan_classes(const s2tc_amd_job *jobs, int njobs, std::vector<ClassPlan> &cl
 			c.all16 = all16;
 			c.cd = cd;
 			c.dxt = c.normal ? -1 : dxt;
-			c.refine = fast ? refine : -1;
+			c.refine = (fast || all16) ? refine : -1;
 			c.tile = c.normal ? 4 : 256;
 		}
 		ClassPlan &c = classes[slot[key]];
@@ -77,7 +78,7 @@ int plan_classes(const s2tc_amd_job *jobs, int njobs, std::vector<ClassPlan> &cl
 	}
 	for (auto &c : classes) {
 		const KernelTable &kt = kernel_table(c.cd);
-		c.kernel = c.normal ? kt.normal_batch : c.all16 ? kt.all16_batch[c.dxt] : kt.fast_batch[c.dxt][c.refine];
+		c.kernel = c.normal ? kt.normal_batch : c.all16 ? kt.all16_batch[c.dxt][c.refine] : kt.fast_batch[c.dxt][c.refine];
 		if (!c.kernel) return set_error(S2TC_E_ARG, "no batch kernel for distance %d", c.cd);
 		c.ntiles = (int)((c.nblocks + c.tile - 1) / c.tile);
 		c.smem = c.normal ? (size_t)kt.normal_smem_per_cand * 16 * c.pbw_max : 0;

@@ -11,15 +11,15 @@ namespace s2tc {
 //   quick mode (nrandom < 0, distance != NORMALMAP): one class per
 //     (distance, format, refinement), k_fast_batch, one lane per block,
 //     tiles of 256;
-//   MODE_NORMAL without random colours: one class per (distance, format),
-//     k_all16_batch, one lane per block, tiles of 256;
+//   MODE_NORMAL without random colours: one class per (distance, format,
+//     refinement), k_all16_batch, one lane per block, tiles of 256;
 //   MODE_NORMAL with random colours: one class per distance (formats mixed),
 //     k_normal_batch, 16 lanes per block, tiles of 4 blocks.
 struct ClassPlan {
 	bool normal = false;         // 16-lanes-per-block kernel
 	bool all16 = false;          // one-lane-per-block MODE_NORMAL kernel
 	int cd = 0, dxt = -1;
-	int refine = -1;             // quick-mode classes: the refinement all their jobs use
+	int refine = -1;             // one-lane classes: the refinement all their jobs use
 	std::vector<JobDesc> jobs;
 	std::vector<TileJobs> tiles;
 	std::vector<int> job_index;  // index into the caller's job array
