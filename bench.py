@@ -32,6 +32,20 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def measured_traffic():
+    """HBM bytes per launch per kernel from the committed rocprofv3 counter
+    summary (profiles/traffic_per_launch.txt): {kernel: (read, written)}."""
+    out = {}
+    path = os.path.join(ROOT, "profiles", "traffic_per_launch.txt")
+    if os.path.exists(path):
+        for line in open(path):
+            if line.startswith("#") or not line.strip():
+                continue
+            k, rd, wr = line.rstrip("\n").split("\t")
+            out[k] = (int(rd), int(wr))
+    return out
+
+
 def batch_items(seed, count, serve_mix=False):
     from s2tc_amd import workloads
     jobs = workloads.batch_jobs(seed, count, serve_mix)
@@ -149,10 +163,12 @@ def quick_leg(s, size=16384, reps=10):
         texels = size * size
         nbytes = 4 * texels + nb * s.block_bytes(dxt)
         gbs = nbytes / (ms * 1e-3) / 1e9
-        out.append(dict(kernel=f"k_fast<WAVG, {('DXT1', 'DXT3', 'DXT5')[dxt]}, rgba8>", image=f"{size}x{size}",
-                        dxt=(1, 3, 5)[dxt], texels=texels, kernel_ms=round(ms, 4),
-                        mtexels_s=round(texels / (ms * 1e-3) / 1e6, 1), algorithmic_bytes=nbytes,
-                        achieved_gbs=round(gbs, 1), frac=round(gbs / HBM_PEAK_GBS, 4)))
+        kname = f"k_fast_interior<5, {dxt}, true, 1>"
+        tr = measured_traffic().get(kname) if size == 16384 else None
+        out.append(dict(kernel=kname, image=f"{size}x{size}", dxt=(1, 3, 5)[dxt], texels=texels,
+                        kernel_ms=round(ms, 4), mtexels_s=round(texels / (ms * 1e-3) / 1e6, 1),
+                        algorithmic_bytes=nbytes, achieved_gbs=round(gbs, 1), frac=round(gbs / HBM_PEAK_GBS, 4),
+                        traffic=sum(tr) if tr else None))
         log(f"quick {out[-1]}")
         d_dst.free()
     # a plain streaming copy of the same source, for the achievable read+write rate
@@ -225,6 +241,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-quick", action="store_true")
     ap.add_argument("--quick-size", type=int, default=16384)
+    ap.add_argument("--only-quick", action="store_true",
+                    help="only run the quick-mode leg (one large image) and print its JSON")
     ap.add_argument("--serve-duration", type=float, default=0.5)
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="only run a plain device copy of this many MiB (PMC calibration) and exit")
@@ -246,6 +264,10 @@ def main():
             s.check(L.s2tc_amd_memcpy_d2d_kernel(dst.ptr, src.ptr, n, None))
         s.synchronize()
         print(json.dumps({"calibrate_copy_bytes": n}))
+        return
+
+    if a.only_quick:
+        print(json.dumps({"quick_mode": quick_leg(s, a.quick_size, max(1, a.steps))}), flush=True)
         return
 
     t0 = time.time()
@@ -284,12 +306,15 @@ def main():
                 c["ms"] = min(c["ms"], n["ms"])
     dom = max(classes, key=lambda c: c["ms"])
     names = s.COLORDIST_NAMES
-    kname = {0: f"k_fast_batch<{dom['cd']}, {dom['dxt']}>", 1: f"k_normal_batch<{dom['cd']}>",
-             2: f"k_all16_batch<{dom['cd']}, {dom['dxt']}>"}[dom["normal"]]
+    kname = {0: f"k_fast_batch<{dom['cd']}, {dom['dxt']}, {dom['refine']}>", 1: f"k_normal_batch<{dom['cd']}>",
+             2: f"k_all16_batch<{dom['cd']}, {dom['dxt']}, {dom['refine']}>"}[dom["normal"]]
     gbs = dom["bytes"] / (dom["ms"] * 1e-3) / 1e9
+    tr = measured_traffic().get(kname) if a.workload == "batch" else None
     roofline = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kname,
+                "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": sum(tr) if tr else None, "kernel": kname,
                 "kernel_ms": round(dom["ms"], 4), "algorithmic_bytes": dom["bytes"]}
+    if tr:
+        roofline["traffic_source"] = "profiles/traffic_per_launch.txt (rocprofv3 FETCH_SIZE x2, WRITE_SIZE x1)"
     if dom["normal"] == 2:
         # pair-scan floor of reduce_colors_inplace: n m (m - 1) min+add lane-ops
         # per block with n = m = 16, twice for DXT5 alpha
@@ -299,6 +324,7 @@ def main():
                             "unit": "Tlane-op/s", "frac": round(tops / VALU_PEAK_TOPS, 4),
                             "algorithmic_ops": ops}
     class_summary = [dict(kernel={0: "fast", 1: "normal", 2: "all16"}[c["normal"]], cd=names[c["cd"]], dxt=c["dxt"],
+                          refine=c["refine"],
                           jobs=c["njobs"], texels=c["texels"], ms=round(c["ms"], 4),
                           gbs=round(c["bytes"] / (c["ms"] * 1e-3) / 1e9, 1)) for c in classes]
 

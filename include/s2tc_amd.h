@@ -86,6 +86,7 @@ typedef struct {
 	int normal;        /* 0: quick mode, 1: MODE_NORMAL with random colours (16 lanes
 	                      per block), 2: MODE_NORMAL with the block colours only */
 	int cd, dxt;       /* dxt is -1 for kind 1 classes (mixed formats) */
+	int refine;        /* the refinement of all jobs of a kind 0 or 2 class, -1 for kind 1 */
 	int njobs;
 	long long blocks, texels;
 	long long bytes;   /* algorithmic HBM bytes: 4 per texel read + block bytes written */
@@ -94,8 +95,9 @@ typedef struct {
 
 typedef struct s2tc_amd_batch s2tc_amd_batch;
 
-/* Plans a batch: jobs are grouped into kernel classes (quick mode per
- * distance and format, MODE_NORMAL per distance); each class runs one
+/* Plans a batch: jobs are grouped into kernel classes (quick mode and
+ * MODE_NORMAL without random colours per distance, format and refinement,
+ * MODE_NORMAL with random colours per distance); each class runs one
  * persistent kernel over a packed list of 4x4 blocks, and all classes are
  * captured into one hipGraph.  The job array is copied; the device buffers
  * it points to must stay valid while the batch is used. */

@@ -143,7 +143,7 @@ class Job(ctypes.Structure):
 
 class ClassTime(ctypes.Structure):
     """s2tc_amd_class_time"""
-    _fields_ = [("normal", _I), ("cd", _I), ("dxt", _I), ("njobs", _I), ("blocks", ctypes.c_longlong),
+    _fields_ = [("normal", _I), ("cd", _I), ("dxt", _I), ("refine", _I), ("njobs", _I), ("blocks", ctypes.c_longlong),
                 ("texels", ctypes.c_longlong), ("bytes", ctypes.c_longlong), ("ms", ctypes.c_double)]
 
 
@@ -240,7 +240,8 @@ class Batch:
         n = lib().s2tc_amd_batch_class_times(self.h, stream, ctypes.cast(arr, _P), 64)
         if n < 0:
             check(n)
-        return [dict(normal=c.normal, cd=c.cd, dxt=c.dxt, njobs=c.njobs, blocks=c.blocks, texels=c.texels,
+        return [dict(normal=c.normal, cd=c.cd, dxt=c.dxt, refine=c.refine, njobs=c.njobs, blocks=c.blocks,
+                     texels=c.texels,
                      bytes=c.bytes, ms=c.ms) for c in arr[:n]]
 
     def destroy(self):

@@ -227,6 +227,7 @@ int s2tc_amd_batch_class_times(s2tc_amd_batch *b, void *stream, s2tc_amd_class_t
 			o.normal = c.normal ? 1 : c.all16 ? 2 : 0;
 			o.cd = c.cd;
 			o.dxt = c.dxt;
+			o.refine = c.refine;
 			o.njobs = (int)c.jobs.size();
 			o.blocks = c.nblocks;
 			o.texels = c.texels;
