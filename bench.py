@@ -191,6 +191,39 @@ def quick_leg(s, size=16384, reps=10):
     return out
 
 
+def quality_leg(s, w=1024, h=512):
+    """Total colour distance (s2tc_amd_colordist_error) of a NORMALMAP,
+    nrandom 64, REFINE_LOOP, DXT1 encoding of a w x h normal map (512
+    Gaussians, as the 8192x8192 case), with this encoder's counter-based
+    generator and with the unmodified reference drawing from libc's rand()
+    (oracle/_ref/libs2tc_ref_stock.so, single thread, srand(1))."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle import StockReference, stock_reference_available
+    if not stock_reference_available():
+        return None
+    import numpy as np
+    from s2tc_amd import synth
+    rgba = synth.normalmap(w, h, 2, 512, 4)
+    q = rgba.reshape(-1, 4).copy()
+    q[:, 0] >>= 3
+    q[:, 1] >>= 2
+    q[:, 2] >>= 3
+    q[:, 3] >>= 7
+    ours = s.encode_image(rgba, w, h, 0, s.NORMALMAP, 64, s.REFINE_LOOP, seed=2, src_kind=s.SRC_RGBA8)
+    e_ours = s.colordist_error(rgba, w, h, ours, 0, s.NORMALMAP, src_kind=s.SRC_RGBA8)
+    st = StockReference()
+    st.srand(1)
+    t0 = time.perf_counter()
+    stock = st.encode_quantized(q.reshape(-1), w, h, 0, s.NORMALMAP, 64, s.REFINE_LOOP)
+    t_stock = time.perf_counter() - t0
+    e_stock = s.colordist_error(rgba, w, h, stock, 0, s.NORMALMAP, src_kind=s.SRC_RGBA8)
+    out = dict(workload=f"{w}x{h} normal map, NORMALMAP, nrandom 64, LOOP, DXT1", colordist_error=e_ours,
+               stock_reference_colordist_error=e_stock, ratio=round(e_ours / max(e_stock, 1), 5),
+               stock_reference_seconds=round(t_stock, 2))
+    log(f"quality {out}")
+    return out
+
+
 def serve_leg(s, rates, duration, npool=256, seed=4):
     """Open-loop Poisson arrivals of textures (with mip chains) drawn from a
     pool of the serving mix (70% quick mode, 30% nrandom 64 + LOOP), sources
@@ -350,10 +383,13 @@ def main():
         serve = serve_leg(s, (2000, 10000, 50000), a.serve_duration)
 
     cpu = None
+    quality = None
     if not a.no_cpu:
         cpu = cpu_baseline(items)
         if cpu:
             cpu["value"] = round(cpu["value"], 2)
+        if a.workload == "batch":
+            quality = quality_leg(s)
 
     out = {
         "metric": "Mtexels/s bit-exact, batched persistent vs per-texture plain; p99 serve latency",
@@ -370,6 +406,7 @@ def main():
         "quick_mode": quick,
         "cpu_baseline": cpu,
         "serve": serve,
+        "quality": quality,
     }
     print(json.dumps(out), flush=True)
     batch.destroy()

@@ -117,6 +117,24 @@ S2TC_API int s2tc_amd_batch_launch(s2tc_amd_batch *b, void *stream);
 S2TC_API int s2tc_amd_batch_class_times(s2tc_amd_batch *b, void *stream, s2tc_amd_class_time *out, int max_out);
 S2TC_API int s2tc_amd_batch_destroy(s2tc_amd_batch *b);
 
+/* Decoding and encoding error (no counterpart among the reference's entry
+ * points; for checking and measuring encodings on the device).
+ *
+ * s2tc_amd_decode_image: every texel of a block image with the rules of
+ * fetch_2d_texel_{rgb_dxt1,rgba_dxt1,rgba_dxt3,rgba_dxt5}
+ * (s2tc_libtxc_dxtn.cpp:35-140); format is the GL enum 0x83F0..0x83F3;
+ * d_rgba receives width*height RGBA8 texels.  Asynchronous on `stream`.
+ *
+ * s2tc_amd_colordist_error: the sum over texels of ColorDist(texel, decoded
+ * colour) in distance mode cd - the per-texel quantity the encoder minimises
+ * (s2tc_algorithm.cpp:606-616) - for a source image (S2TC_SRC_QUANTIZED or
+ * S2TC_SRC_RGBA8 quantized with DITHER_NONE) and its blocks; DXT1 texels with
+ * alpha 0 are skipped.  Synchronous: *total is set on return. */
+S2TC_API int s2tc_amd_decode_image(const void *d_blocks, unsigned int format, int width, int height, void *d_rgba,
+                                   void *stream);
+S2TC_API int s2tc_amd_colordist_error(const void *d_src, int src_kind, int width, int height, const void *d_blocks,
+                                      int dxt, int cd, long long *total, void *stream);
+
 /* The same jobs with one plain kernel launch per job (the per-texture path). */
 S2TC_API int s2tc_amd_encode_jobs_plain(const s2tc_amd_job *jobs, int njobs, void *stream);
 

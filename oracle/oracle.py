@@ -6,7 +6,9 @@ this module.  Two checkers:
 * ``Oracle``    - liboracle.so, the C restatement in s2tc_oracle.c;
 * ``Reference`` - _ref/libs2tc_ref.so, the unmodified reference sources
   compiled with ref_harness.cpp (present where it was built; it travels with
-  the tree as a built file).
+  the tree as a built file);
+* ``StockReference`` - _ref/libs2tc_ref_stock.so, the same build drawing its
+  random candidates from libc's rand(), for comparing encoding error.
 """
 import ctypes
 import os
@@ -55,6 +57,8 @@ class Oracle(_Base):
         L.oracle_mip_reduce.argtypes = [P, P, ctypes.POINTER(I), ctypes.POINTER(I)]
         L.oracle_color_dist.argtypes = [I] * 7
         L.oracle_srgb_get_y.argtypes = [I, I, I]
+        L.oracle_colordist_error.argtypes = [P, I, I, I, I, P]
+        L.oracle_colordist_error.restype = ctypes.c_int64
         self._encode = L.oracle_encode_image_quantized
 
     def rgb565_image(self, rgba, width, height, srccomps, alphabits, dither):
@@ -91,6 +95,12 @@ class Oracle(_Base):
     def color_dist(self, cd, a, b):
         return self.lib.oracle_color_dist(cd, *a, *b)
 
+    def colordist_error(self, q, width, height, dxt, cd, blocks):
+        """Sum over texels of ColorDist(texel, decoded colour) (DXT1: alpha-0 texels skipped)."""
+        q = np.ascontiguousarray(q, dtype=np.uint8)
+        blocks = np.ascontiguousarray(blocks, dtype=np.uint8)
+        return int(self.lib.oracle_colordist_error(_ptr(q), width, height, dxt, cd, _ptr(blocks)))
+
 
 class Reference(_Base):
     """The compiled reference encoder (oracle/_ref), if it was built."""
@@ -121,6 +131,27 @@ class Reference(_Base):
         out = np.zeros(width * height * 4, np.uint8)
         self.lib.rgb565_image(_ptr(out), _ptr(rgba), width, height, srccomps, alphabits, dither)
         return out
+
+
+class StockReference(_Base):
+    """The compiled reference with libc's rand() (oracle/_ref/libs2tc_ref_stock.so).
+    Single-threaded: the draws then follow the reference's own block order."""
+
+    def __init__(self, path=None):
+        path = path or os.path.join(HERE, "_ref", "libs2tc_ref_stock.so")
+        if not os.path.exists(path):
+            raise FileNotFoundError(path)
+        L = self.lib = ctypes.CDLL(path)
+        L.ref_encode_image_quantized.argtypes = [P, I, I, I, I, I, I, U64, P, I, I]
+        L.ref_stock_srand.argtypes = [ctypes.c_uint]
+        self._encode = L.ref_encode_image_quantized
+
+    def srand(self, seed=1):
+        self.lib.ref_stock_srand(seed)
+
+
+def stock_reference_available():
+    return os.path.exists(os.path.join(HERE, "_ref", "libs2tc_ref_stock.so"))
 
 
 def reference_available():

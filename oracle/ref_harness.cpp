@@ -13,6 +13,8 @@
  *      per-call    inside tx_compress_dxtn the k-th call belongs to block
  *                  k / dpb (raster order), draw k % dpb, with
  *                  dpb = nrandom * (DXT5 ? 4 : 3) (s2tc_algorithm.cpp:984-992).
+ *    Built with -DS2TC_STOCK_RAND (libs2tc_ref_stock.so) the override is
+ *    left out and the reference draws from libc's rand().
  *  - tx_compress_dxtn: a wrapper that sets per-call mode (seed from the
  *    S2TC_RANDOM_SEED environment variable, default 0) and calls the
  *    reference's function, compiled under the name s2tc_ref_tx_compress_dxtn.
@@ -49,6 +51,11 @@ thread_local RandState g_rs;
 std::atomic<uint64_t> g_rand_calls{0};
 }
 
+#ifdef S2TC_STOCK_RAND
+/* The stock build keeps libc's rand() (libs2tc_ref_stock.so, for comparing
+ * encoding error with the unmodified generator); only its seed is exposed. */
+extern "C" void ref_stock_srand(unsigned int s) { srand(s); }
+#else
 extern "C" int rand(void)
 {
 	g_rand_calls.fetch_add(1, std::memory_order_relaxed);
@@ -61,6 +68,7 @@ extern "C" int rand(void)
 	}
 	return s2tc_prng_draw(s.bkey, (uint32_t)s.counter++);
 }
+#endif
 
 extern "C" uint64_t ref_rand_calls(void) { return g_rand_calls.load(); }
 

@@ -788,6 +788,31 @@ void oracle_fetch_texel(unsigned fmt, int srcRowStride, const uint8_t *pix, int 
 	}
 }
 
+/* ---- total colour distance of an encoding (test metric) ---- */
+int64_t oracle_colordist_error(const uint8_t *q, int width, int height, int dxt, int cd, const uint8_t *blocks)
+{
+	const int bw = (width + 3) / 4, bs = dxt == O_DXT1 ? 8 : 16;
+	int64_t total = 0;
+	int i, j;
+	for (j = 0; j < height; ++j)
+		for (i = 0; i < width; ++i) {
+			const uint8_t *t = q + ((size_t)j * width + i) * 4;
+			const uint8_t *cb = blocks + ((size_t)(j >> 2) * bw + (i >> 2)) * bs + (dxt == O_DXT1 ? 0 : 8);
+			const unsigned c0 = cb[0] + 256u * cb[1], c1 = cb[2] + 256u * cb[3];
+			const int b = (cb[4 + (j & 3)] >> (2 * (i & 3))) & 3;
+			unsigned c = c0;
+			ocol tc, dc;
+			if (dxt == O_DXT1 && t[3] == 0) continue;
+			if (b == 1) c = c1;
+			else if (dxt == O_DXT1 && b == 3 && c1 >= c0) c = 0;
+			else if (b >= 2 && ((i ^ j) & 1)) c = c1;
+			tc.r = sc(t[0]); tc.g = sc(t[1]); tc.b = sc(t[2]);
+			dc.r = sc((int)(c >> 11) & 31); dc.g = sc((int)(c >> 5) & 63); dc.b = sc((int)c & 31);
+			total += cdist(cd, tc, dc);
+		}
+	return total;
+}
+
 /* ---- Image_MipReduce32 toward 1x1 (s2tc_compress.c:427-493) ---- */
 void oracle_mip_reduce(const uint8_t *in, uint8_t *out, int *width, int *height)
 {

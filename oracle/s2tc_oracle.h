@@ -61,6 +61,14 @@ void oracle_encode_image_quantized(const uint8_t *q, int width, int height, int 
 void oracle_fetch_texel(unsigned fmt, int srcRowStride, const uint8_t *pix,
                         int i, int j, uint8_t *texel);
 
+/* Total colour distance of an encoding: the sum over texels of
+ * ColorDist(texel, colour the block decodes to), the per-texel quantity
+ * s2tc_try_encode_block minimises (s2tc_algorithm.cpp:606-616), with the
+ * decoded 5:6:5 colour chosen by the fetch_2d_texel rules
+ * (s2tc_libtxc_dxtn.cpp:35-140).  DXT1 texels with alpha 0 are skipped.
+ * q: quantized {r5, g6, b5, a} texels; blocks in raster order. */
+int64_t oracle_colordist_error(const uint8_t *q, int width, int height, int dxt, int cd, const uint8_t *blocks);
+
 /* Image_MipReduce32 to (destwidth, destheight) = (1, 1); in may equal out. */
 void oracle_mip_reduce(const uint8_t *in, uint8_t *out, int *width, int *height);
 

@@ -50,6 +50,8 @@ def lib():
     L.s2tc_amd_memcpy_d2h.argtypes = [_P, _P, _SZ]
     L.s2tc_amd_memset.argtypes = [_P, _I, _SZ]
     L.s2tc_amd_encode_image.argtypes = [_P, _I, _I, _I, _P, _I, _I, _I, _I, _U64, _P]
+    L.s2tc_amd_decode_image.argtypes = [_P, ctypes.c_uint, _I, _I, _P, _P]
+    L.s2tc_amd_colordist_error.argtypes = [_P, _I, _I, _I, _P, _I, _I, ctypes.POINTER(ctypes.c_longlong), _P]
     _lib = L
     return L
 
@@ -131,6 +133,48 @@ def encode_image(texels, width, height, dxt, cd=WAVG, nrandom=-1, refine=REFINE_
     src.free()
     dst.free()
     return out
+
+
+def _dptr(b):
+    return b.ptr if isinstance(b, DeviceBuffer) else b
+
+
+def decode_image_device(d_blocks, fmt, width, height, d_rgba, stream=None):
+    """fetch_2d_texel_* of every texel into d_rgba (width*height RGBA8), asynchronous."""
+    check(lib().s2tc_amd_decode_image(_dptr(d_blocks), fmt, width, height, _dptr(d_rgba), stream))
+
+
+def decode_image(blocks, fmt, width, height):
+    """Host convenience: decode a block image on the GPU -> RGBA8 bytes."""
+    blocks = np.ascontiguousarray(blocks, dtype=np.uint8).reshape(-1)
+    d_b = DeviceBuffer.from_host(blocks)
+    d_o = DeviceBuffer(width * height * 4)
+    decode_image_device(d_b, fmt, width, height, d_o)
+    synchronize()
+    out = d_o.to_host(width * height * 4)
+    d_b.free()
+    d_o.free()
+    return out
+
+
+def colordist_error_device(d_src, src_kind, width, height, d_blocks, dxt, cd, stream=None):
+    """Sum over texels of ColorDist(texel, decoded colour) (synchronous)."""
+    total = ctypes.c_longlong(0)
+    check(lib().s2tc_amd_colordist_error(_dptr(d_src), src_kind, width, height, _dptr(d_blocks), dxt, cd,
+                                         ctypes.byref(total), stream))
+    return total.value
+
+
+def colordist_error(texels, width, height, blocks, dxt, cd, src_kind=SRC_QUANTIZED):
+    """Host convenience for colordist_error_device."""
+    texels = np.ascontiguousarray(texels, dtype=np.uint8).reshape(-1)
+    blocks = np.ascontiguousarray(blocks, dtype=np.uint8).reshape(-1)
+    d_s, d_b = DeviceBuffer.from_host(texels[:width * height * 4]), DeviceBuffer.from_host(blocks)
+    try:
+        return colordist_error_device(d_s, src_kind, width, height, d_b, dxt, cd)
+    finally:
+        d_s.free()
+        d_b.free()
 
 
 # ---- batches ----

@@ -1,0 +1,113 @@
+// decode.hip - S2TC block decoding and the total colour distance of an
+// encoding, on the device.
+//
+// k_decode follows fetch_2d_texel_* (s2tc_libtxc_dxtn.cpp:35-140) for every
+// texel of an image: index 0 / 1 take c0 / c1, indices 2 and 3 take the
+// checkerboard of c0 and c1 (i ^ j odd -> c1), except DXT1 index 3 with
+// c1 >= c0, which is black (alpha 0 for RGBA); 5:6:5 expands as (v << 3) |
+// (v >> 2) and (v << 2) | (v >> 4); DXT3 alpha a | a << 4; DXT5 alpha in
+// the 6-value mode (a1 >= a0: 6 -> 0, 7 -> 255, 6 falling through to 7's
+// test otherwise) else the checkerboard.  One thread per texel.
+//
+// The colour-distance kernels live with the encoders (kernels_cd.hip),
+// one per distance; s2tc_amd_colordist_error dispatches to them.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "s2tc_internal.h"
+
+namespace s2tc {
+
+__device__ __forceinline__ uint32_t expand565(uint32_t c)
+{
+	const uint32_t r = (c >> 11) & 31u, g = (c >> 5) & 63u, b = c & 31u;
+	return ((r << 3) | (r >> 2)) | (((g << 2) | (g >> 4)) << 8) | (((b << 3) | (b >> 2)) << 16);
+}
+
+__global__ __launch_bounds__(256) void k_decode(const uint8_t *__restrict__ blocks, unsigned fmt, int width,
+                                                int height, uint32_t *__restrict__ rgba)
+{
+	const int i = (int)(blockIdx.x * 64 + threadIdx.x), j = (int)(blockIdx.y * 4 + threadIdx.y);
+	if (i >= width || j >= height) return;
+	const int bpb = (fmt == 0x83F0u || fmt == 0x83F1u) ? 8 : 16;
+	const uint8_t *blk = blocks + ((size_t)((width + 3) >> 2) * (size_t)(j >> 2) + (size_t)(i >> 2)) * bpb;
+	const uint8_t *cb = bpb == 8 ? blk : blk + 8;
+	uint32_t c = cb[0] | (cb[1] << 8);
+	const uint32_t c1 = cb[2] | (cb[3] << 8);
+	const int b = (cb[4 + (j & 3)] >> (2 * (i & 3))) & 3;
+	uint32_t a = 255;
+	const bool odd = ((i ^ j) & 1) != 0;
+	if (b == 1) {
+		c = c1;
+	} else if (bpb == 8 && b == 3 && c1 >= c) {
+		c = 0;
+		if (fmt == 0x83F1u) a = 0;
+	} else if (b >= 2 && odd) {
+		c = c1;
+	}
+	if (fmt == 0x83F2u) {
+		const uint32_t q = (blk[(j & 3) * 2 + ((i & 3) >> 1)] >> (4 * (i & 1))) & 15u;
+		a = q | (q << 4);
+	} else if (fmt == 0x83F3u) {
+		a = blk[0];
+		const uint32_t a1 = blk[1];
+		uint64_t bits = 0;
+#pragma unroll
+		for (int q = 0; q < 6; ++q) bits |= (uint64_t)blk[2 + q] << (8 * q);
+		const int ab = (int)((bits >> (3 * ((j & 3) * 4 + (i & 3)))) & 7u);
+		if (ab == 1) a = a1;
+		else if (ab == 6 && a1 >= a) a = 0;
+		else if ((ab == 6 || ab == 7) && a1 >= a) a = 255;
+		else if (ab != 0 && odd) a = a1;
+	}
+	rgba[(size_t)j * (size_t)width + (size_t)i] = expand565(c) | (a << 24);
+}
+
+}  // namespace s2tc
+
+using namespace s2tc;
+
+extern "C" int s2tc_amd_decode_image(const void *d_blocks, unsigned int format, int width, int height, void *d_rgba,
+                                     void *stream)
+{
+	if (!d_blocks || !d_rgba || width <= 0 || height <= 0)
+		return set_error(S2TC_E_ARG, "decode: bad arguments");
+	if (format < 0x83F0u || format > 0x83F3u) return set_error(S2TC_E_ARG, "decode: bad format 0x%x", format);
+	if (int r = ensure_device()) return r;
+	const int by = (height + 3) / 4;
+	if (by > 65535) return set_error(S2TC_E_RANGE, "decode: image too tall");
+	dim3 grid((unsigned)((width + 63) / 64), (unsigned)by), block(64, 4);
+	k_decode<<<grid, block, 0, (hipStream_t)stream>>>((const uint8_t *)d_blocks, format, width, height,
+	                                                   (uint32_t *)d_rgba);
+	return hip_check(hipGetLastError(), "launch k_decode");
+}
+
+extern "C" int s2tc_amd_colordist_error(const void *d_src, int src_kind, int width, int height, const void *d_blocks,
+                                        int dxt, int cd, long long *total, void *stream)
+{
+	if (!d_src || !d_blocks || !total || width <= 0 || height <= 0)
+		return set_error(S2TC_E_ARG, "colordist_error: bad arguments");
+	if (src_kind != S2TC_SRC_QUANTIZED && src_kind != S2TC_SRC_RGBA8)
+		return set_error(S2TC_E_ARG, "colordist_error: bad source kind %d", src_kind);
+	int refine = REFINE_ALWAYS;
+	normalize_modes(dxt, cd, refine);
+	if (int r = ensure_device()) return r;
+	const KernelTable &kt = kernel_table(cd);
+	const void *k = kt.error[dxt][src_kind == S2TC_SRC_RGBA8 ? 1 : 0];
+	if (!k) return set_error(S2TC_E_ARG, "colordist_error: no kernel for distance %d", cd);
+	const int bw = (width + 3) / 4, bh = (height + 3) / 4;
+	if (bh > 4 * 65535) return set_error(S2TC_E_RANGE, "colordist_error: image too tall");
+	hipStream_t st = (hipStream_t)stream;
+	long long *d_sum = nullptr;
+	int rc = hip_check(hipMalloc((void **)&d_sum, sizeof(long long)), "hipMalloc");
+	if (!rc) rc = hip_check(hipMemsetAsync(d_sum, 0, sizeof(long long), st), "hipMemsetAsync");
+	if (!rc) {
+		void *args[] = {(void *)&d_src, (void *)&width, (void *)&height, (void *)&d_blocks, (void *)&d_sum};
+		dim3 grid((unsigned)((bw + 63) / 64), (unsigned)((bh + 3) / 4)), block(64, 4);
+		rc = hip_check(hipLaunchKernel(k, grid, block, args, 0, st), "launch colour-distance kernel");
+	}
+	if (!rc) rc = hip_check(hipMemcpyAsync(total, d_sum, sizeof(long long), hipMemcpyDeviceToHost, st), "D2H");
+	if (!rc) rc = hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+	if (d_sum) (void)hipFree(d_sum);
+	return rc;
+}

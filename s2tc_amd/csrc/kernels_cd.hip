@@ -299,6 +299,48 @@ __global__ __launch_bounds__(64) void k_normal_batch(const JobDesc *__restrict__
 	}
 }
 
+// Total colour distance of an encoding, one lane per block: the sum over
+// the block's texels of ColorDist(texel, decoded colour) - the quantity
+// s2tc_try_encode_block minimises (s2tc_algorithm.cpp:606-616) - with the
+// colour decoded as fetch_2d_texel_* does (s2tc_libtxc_dxtn.cpp:35-140):
+// index 0 / 1 -> c0 / c1, 2 and 3 -> checkerboard, DXT1 index 3 with
+// c1 >= c0 -> black.  DXT1 texels with alpha 0 are skipped.  Each workgroup
+// adds its int64 sum to *sum with one atomic.
+template <int CD, int DXT, bool RAW>
+__global__ __launch_bounds__(256) void k_colordist_error(const uint32_t *__restrict__ src, int width, int height,
+                                                         const uint8_t *__restrict__ blocks, long long *sum)
+{
+	typedef Dist<CD> D;
+	const int bw = (width + 3) >> 2, bh = (height + 3) >> 2;
+	const int bx = (int)(blockIdx.x * 64 + threadIdx.x), by = (int)(blockIdx.y * 4 + threadIdx.y);
+	long long e = 0;
+	if (bx < bw && by < bh) {
+		const int w = min(4, width - 4 * bx), h = min(4, height - 4 * by);
+		const uint8_t *cb = blocks + ((size_t)by * (size_t)bw + (size_t)bx) * (DXT == DXT1 ? 8 : 16) + (DXT == DXT1 ? 0 : 8);
+		const uint32_t c0 = cb[0] | (cb[1] << 8), c1 = cb[2] | (cb[3] << 8);
+		const typename D::F f0 = D::feat(c0), f1 = D::feat(c1), fk = D::feat(0u);
+		for (int y = 0; y < h; ++y) {
+			const uint32_t row = cb[4 + y];
+			for (int x = 0; x < w; ++x) {
+				uint32_t t = src[(size_t)(4 * by + y) * (size_t)width + 4 * bx + x];
+				if (RAW) t = quant_none<DXT>(t);
+				if (DXT == DXT1 && texel_a(t) == 0) continue;
+				const int b = (int)((row >> (2 * x)) & 3u);
+				const bool odd = ((x ^ y) & 1) != 0;  // (i ^ j) & 1 with i, j multiples of 4 apart
+				const typename D::F &fd = b == 0 ? f0 : b == 1 ? f1 : (DXT == DXT1 && b == 3 && c1 >= c0) ? fk : odd ? f1 : f0;
+				e += D::d(D::featq(t), fd);
+			}
+		}
+	}
+#pragma unroll
+	for (int o = 32; o > 0; o >>= 1) e += __shfl_down(e, o, 64);
+	__shared__ long long part[4];
+	const int lane = (int)(threadIdx.y * 64 + threadIdx.x);
+	if ((lane & 63) == 0) part[lane >> 6] = e;
+	__syncthreads();
+	if (lane == 0) atomicAdd((unsigned long long *)sum, (unsigned long long)(part[0] + part[1] + part[2] + part[3]));
+}
+
 #ifdef S2TC_SRGB_Y_TABLE
 __global__ __launch_bounds__(256) void k_srgb_y_table_init(int)
 {
@@ -356,6 +398,12 @@ extern "C" void S2TC_CAT(s2tc_kernels_cd, S2TC_CD)(KernelTable *kt)
 	for (int d = 0; d < 3; ++d) kt->fast_batch[d][0] = kt->fast_batch[d][1] = kt->fast_batch[d][2] = nullptr;
 #endif
 	kt->normal_batch = (const void *)k_normal_batch<S2TC_CD>;
+	kt->error[DXT1][0] = (const void *)k_colordist_error<S2TC_CD, DXT1, false>;
+	kt->error[DXT3][0] = (const void *)k_colordist_error<S2TC_CD, DXT3, false>;
+	kt->error[DXT5][0] = (const void *)k_colordist_error<S2TC_CD, DXT5, false>;
+	kt->error[DXT1][1] = (const void *)k_colordist_error<S2TC_CD, DXT1, true>;
+	kt->error[DXT3][1] = (const void *)k_colordist_error<S2TC_CD, DXT3, true>;
+	kt->error[DXT5][1] = (const void *)k_colordist_error<S2TC_CD, DXT5, true>;
 	kt->all16[DXT1][0] = (const void *)k_all16<S2TC_CD, DXT1, false>;
 	kt->all16[DXT3][0] = (const void *)k_all16<S2TC_CD, DXT3, false>;
 	kt->all16[DXT5][0] = (const void *)k_all16<S2TC_CD, DXT5, false>;

@@ -230,3 +230,45 @@ def test_reduce_is_brute_force_argmin(oracle):
         out = oracle.encode_quantized(q, 4, 4, 1, 5, 0, 0)
         c0, c1, _ = _unpack(out, 1)
         assert (c0, c1) == (v0, v1)
+
+
+def test_colordist_error_brute_force(oracle):
+    """oracle_colordist_error equals a Python sum of oracle color_dist over
+    texels decoded with the oracle's fetch (5:6:5 recovered from the 8-bit
+    expansion), on random block bytes, every distance and format."""
+    rng = np.random.default_rng(21)
+    fmt_of = {0: 0x83F1, 1: 0x83F2, 2: 0x83F3}
+    for cd in range(8):
+        for dxt in range(3):
+            w, h = 9, 6
+            q = imgutil.random_quantized(rng, w, h, dxt, "random")
+            nb = ((w + 3) // 4) * ((h + 3) // 4) * (8 if dxt == 0 else 16)
+            blocks = rng.integers(0, 256, nb).astype(np.uint8)
+            total = 0
+            for j in range(h):
+                for i in range(w):
+                    t = q[(j * w + i) * 4:(j * w + i) * 4 + 4]
+                    if dxt == 0 and t[3] == 0:
+                        continue
+                    d = oracle.fetch(fmt_of[dxt], w, blocks, i, j)
+                    dec = (int(d[0]) >> 3, int(d[1]) >> 2, int(d[2]) >> 3)
+                    total += oracle.color_dist(cd, (int(t[0]), int(t[1]), int(t[2])), dec)
+            assert oracle.colordist_error(q, w, h, dxt, cd, blocks) == total, (cd, dxt)
+
+
+def test_colordist_error_zero_for_two_colour_blocks(oracle):
+    """Blocks of at most two distinct colours are encoded without error."""
+    rng = np.random.default_rng(22)
+    w, h = 16, 16
+    q = np.zeros((h, w, 4), np.uint8)
+    for by in range(4):
+        for bx in range(4):
+            cols = [(rng.integers(0, 32), rng.integers(0, 64), rng.integers(0, 32)) for _ in range(2)]
+            for y in range(4):
+                for x in range(4):
+                    q[4 * by + y, 4 * bx + x, :3] = cols[rng.integers(0, 2)]
+    q[..., 3] = 1
+    q = q.reshape(-1)
+    for cd in (4, 5, 6, 0):
+        blocks = oracle.encode_quantized(q, w, h, 0, cd, 0, 1, 0)
+        assert oracle.colordist_error(q, w, h, 0, cd, blocks) == 0, cd
