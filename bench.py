@@ -46,14 +46,6 @@ def measured_traffic():
     return out
 
 
-def batch_items(seed, count, serve_mix=False):
-    from s2tc_amd import workloads
-    jobs = workloads.batch_jobs(seed, count, serve_mix)
-    items = [dict(rgba=lv["rgba"], width=lv["width"], height=lv["height"], dxt=t["dxt"], cd=t["cd"],
-                  nrandom=t["nrandom"], refine=t["refine"], seed=lv["seed"]) for _, _, t, lv in jobs.jobs()]
-    return items
-
-
 def image_items(name):
     from s2tc_amd import workloads
     wl = workloads.IMAGE_WORKLOADS[name]
@@ -67,7 +59,7 @@ def image_items(name):
                  nrandom=wl["nrandom"], refine=wl["refine"], seed=wl["seed"])]
 
 
-def cpu_baseline(items, budget_s=12.0):
+def cpu_baseline(items, budget_s=12.0, imgs=None):
     """The compiled reference (oracle/_ref), threads over all host cores, on a
     sample of the workload's jobs: jobs are taken in a fixed stride order
     until about budget_s of single-thread-equivalent work is reached."""
@@ -97,7 +89,8 @@ def cpu_baseline(items, budget_s=12.0):
         for i in sel:
             it = items[i]
             w, h = it["width"], it["height"]
-            q = ref.rgb565_image(it["rgba"], w, h, 4, {0: 1, 1: 4, 2: 8}[it["dxt"]], 0)
+            rgba = it["rgba"] if it["rgba"] is not None else imgs.source(i)
+            q = ref.rgb565_image(rgba, w, h, 4, {0: 1, 1: 4, 2: 8}[it["dxt"]], 0)
             dst = np.zeros(((w + 3) // 4) * ((h + 3) // 4) * (8 if it["dxt"] == 0 else 16), np.uint8)
             bkeep += [q, dst]
             batch.append(RefJob(q.ctypes.data, dst.ctypes.data, w, h, it["dxt"], it["cd"], it["nrandom"],
@@ -231,15 +224,15 @@ def serve_leg(s, rates, duration, npool=256, seed=4):
     streams vs one-at-a-time plain launches."""
     import numpy as np
     from s2tc_amd import workloads
-    jobs = workloads.batch_jobs(seed, npool, serve_mix=True)
-    items, groups = [], []
-    for t in jobs.textures:
-        f = len(items)
-        for lv in t["levels"]:
-            items.append(dict(rgba=lv["rgba"], width=lv["width"], height=lv["height"], dxt=t["dxt"], cd=t["cd"],
-                              nrandom=t["nrandom"], refine=t["refine"], seed=lv["seed"]))
-        groups.append((f, len(items) - f))
-    imgs = s.DeviceImages(items)
+    textures = workloads.batch_textures(seed, npool, serve_mix=True)
+    imgs = s.DeviceImages.from_textures(textures)
+    groups, f = [], 0
+    for t in textures:
+        n = max(t["width"], t["height"]).bit_length()   # levels down to 1x1
+        groups.append((f, n))
+        f += n
+    assert f == len(imgs.items)
+    del textures
     out = {}
     for mode in ("batched", "serial_plain"):
         for rate in rates:
@@ -304,9 +297,21 @@ def main():
         return
 
     t0 = time.time()
-    items = batch_items(3, 2000) if a.workload == "batch" else image_items(a.workload)
-    log(f"workload {a.workload}: {len(items)} images, generated in {time.time() - t0:.1f} s")
-    imgs = s.DeviceImages(items)
+    mip_leg = None
+    if a.workload == "batch":
+        from s2tc_amd import workloads
+        textures = workloads.batch_textures(3, 2000)
+        log(f"workload batch: {len(textures)} base levels generated in {time.time() - t0:.1f} s")
+        t1 = time.perf_counter()
+        imgs = s.DeviceImages.from_textures(textures)
+        mip_leg = dict(textures=len(textures), uploaded_bytes=imgs.uploaded_bytes,
+                       upload_and_chain_s=round(time.perf_counter() - t1, 3))
+        del textures
+        items = imgs.items
+    else:
+        items = image_items(a.workload)
+        imgs = s.DeviceImages(items)
+    log(f"workload {a.workload}: {len(items)} images on the device after {time.time() - t0:.1f} s")
     texels = imgs.texels
     batch = s.Batch(imgs)
     if a.schedule:
@@ -385,7 +390,7 @@ def main():
     cpu = None
     quality = None
     if not a.no_cpu:
-        cpu = cpu_baseline(items)
+        cpu = cpu_baseline(items, imgs=imgs)
         if cpu:
             cpu["value"] = round(cpu["value"], 2)
         if a.workload == "batch":
@@ -401,6 +406,7 @@ def main():
                    "DXT1/3/5 30/30/40%, 8 distance modes, quick selection, REFINE_ALWAYS, DITHER_NONE",
                    "images": len(items), "texels": texels, "schedule": {"wg_per_cu": sched[0], "streams": sched[1]}},
         "plain_per_texture": plain,
+        "device_mips": mip_leg,
         "roofline": roofline,
         "classes": class_summary,
         "quick_mode": quick,

@@ -135,6 +135,24 @@ S2TC_API int s2tc_amd_decode_image(const void *d_blocks, unsigned int format, in
 S2TC_API int s2tc_amd_colordist_error(const void *d_src, int src_kind, int width, int height, const void *d_blocks,
                                       int dxt, int cd, long long *total, void *stream);
 
+/* The CLI's mip chain on the device: Image_MipReduce32
+ * (s2tc_compress.c:427-493) halves an RGBA8 image with a truncating box
+ * filter, repeated down to 1x1 (s2tc_compress.c:722-733).
+ *
+ * s2tc_amd_mip_layout: number of levels (level 0 = the image) and, for up to
+ * max_levels of them, byte offsets in a chain buffer (each level 256-byte
+ * aligned), sizes and the buffer's total size.
+ * s2tc_amd_mip_reduce: one halving, d_in (width x height) -> d_out.
+ * s2tc_amd_mip_chain: level 0 copied from d_src (no copy when d_src ==
+ * d_levels: the image is already in place), then every level, into d_levels
+ * laid out as s2tc_amd_mip_layout says; returns the level count.
+ * Both are asynchronous on `stream`. */
+S2TC_API int s2tc_amd_mip_layout(int width, int height, int *nlevels, size_t *offsets, int *widths, int *heights,
+                                 int max_levels, size_t *total_bytes);
+S2TC_API int s2tc_amd_mip_reduce(const void *d_in, int width, int height, void *d_out, void *stream);
+S2TC_API int s2tc_amd_mip_chain(const void *d_src, int width, int height, void *d_levels, size_t capacity,
+                                void *stream);
+
 /* The same jobs with one plain kernel launch per job (the per-texture path). */
 S2TC_API int s2tc_amd_encode_jobs_plain(const s2tc_amd_job *jobs, int njobs, void *stream);
 

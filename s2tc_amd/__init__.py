@@ -52,6 +52,10 @@ def lib():
     L.s2tc_amd_encode_image.argtypes = [_P, _I, _I, _I, _P, _I, _I, _I, _I, _U64, _P]
     L.s2tc_amd_decode_image.argtypes = [_P, ctypes.c_uint, _I, _I, _P, _P]
     L.s2tc_amd_colordist_error.argtypes = [_P, _I, _I, _I, _P, _I, _I, ctypes.POINTER(ctypes.c_longlong), _P]
+    L.s2tc_amd_mip_layout.argtypes = [_I, _I, ctypes.POINTER(_I), ctypes.POINTER(_SZ), ctypes.POINTER(_I),
+                                      ctypes.POINTER(_I), _I, ctypes.POINTER(_SZ)]
+    L.s2tc_amd_mip_reduce.argtypes = [_P, _I, _I, _P, _P]
+    L.s2tc_amd_mip_chain.argtypes = [_P, _I, _I, _P, _SZ, _P]
     _lib = L
     return L
 
@@ -177,6 +181,46 @@ def colordist_error(texels, width, height, blocks, dxt, cd, src_kind=SRC_QUANTIZ
         d_b.free()
 
 
+# ---- mip chains ----
+
+def mip_layout(width, height):
+    """[(offset, w, h)] of every level of a device mip chain buffer, and its size in bytes."""
+    L = lib()
+    n, total = _I(0), _SZ(0)
+    check(L.s2tc_amd_mip_layout(width, height, ctypes.byref(n), None, None, None, 0, ctypes.byref(total)))
+    offs, ws, hs = (_SZ * n.value)(), (_I * n.value)(), (_I * n.value)()
+    check(L.s2tc_amd_mip_layout(width, height, ctypes.byref(n), offs, ws, hs, n.value, ctypes.byref(total)))
+    return [(offs[k], ws[k], hs[k]) for k in range(n.value)], total.value
+
+
+def mip_reduce_device(d_in, width, height, d_out, stream=None):
+    """One Image_MipReduce32 halving on the device (asynchronous)."""
+    check(lib().s2tc_amd_mip_reduce(_dptr(d_in), width, height, _dptr(d_out), stream))
+
+
+def mip_chain_device(d_src, width, height, d_levels, capacity, stream=None):
+    """Every level (level 0 = the image) into d_levels as mip_layout places them; returns the level count."""
+    n = lib().s2tc_amd_mip_chain(_dptr(d_src), width, height, _dptr(d_levels), capacity, stream)
+    if n <= 0:
+        check(n if n < 0 else -1)
+    return n
+
+
+def mip_chain(rgba, width, height):
+    """Host convenience: [(w, h, RGBA8 bytes)] of the chain computed on the GPU."""
+    rgba = np.ascontiguousarray(rgba, dtype=np.uint8).reshape(-1)[:width * height * 4]
+    layout, total = mip_layout(width, height)
+    d_s, d_l = DeviceBuffer.from_host(rgba), DeviceBuffer(total)
+    try:
+        mip_chain_device(d_s, width, height, d_l, total)
+        synchronize()
+        buf = d_l.to_host(total)
+    finally:
+        d_s.free()
+        d_l.free()
+    return [(w, h, buf[o:o + w * h * 4].copy()) for o, w, h in layout]
+
+
 # ---- batches ----
 
 class Job(ctypes.Structure):
@@ -215,12 +259,24 @@ class DeviceImages:
 
     def __init__(self, items, src_kind=SRC_RGBA8):
         items = list(items)
-        self.items = items
-        so, do = [], []
-        s_total = d_total = 0
+        so, s_total = [], 0
         for it in items:
             so.append(s_total)
             s_total += (it["width"] * it["height"] * 4 + self.ALIGN - 1) // self.ALIGN * self.ALIGN
+        self._layout(items, so, s_total, src_kind)
+        staging = np.zeros(s_total, np.uint8)
+        for it, o in zip(items, so):
+            a = np.ascontiguousarray(it["rgba"], dtype=np.uint8).reshape(-1)
+            staging[o:o + a.size] = a
+        check(lib().s2tc_amd_memcpy_h2d(self.src.ptr, staging.ctypes.data_as(_P), s_total))
+        del staging
+        self.uploaded_bytes = s_total
+
+    def _layout(self, items, so, s_total, src_kind):
+        """Allocates the source (s_total bytes, image i at so[i]) and output buffers and the jobs."""
+        self.items = items
+        do, d_total = [], 0
+        for it in items:
             do.append(d_total)
             nb = nblocks(it["width"], it["height"]) * block_bytes(it["dxt"])
             d_total += (nb + self.ALIGN - 1) // self.ALIGN * self.ALIGN
@@ -230,16 +286,54 @@ class DeviceImages:
         self.texels = sum(it["width"] * it["height"] for it in items)
         self.bytes_in = self.texels * 4
         self.bytes_out = sum(nblocks(it["width"], it["height"]) * block_bytes(it["dxt"]) for it in items)
-        staging = np.zeros(s_total, np.uint8)
-        for it, o in zip(items, so):
-            a = np.ascontiguousarray(it["rgba"], dtype=np.uint8).reshape(-1)
-            staging[o:o + a.size] = a
-        check(lib().s2tc_amd_memcpy_h2d(self.src.ptr, staging.ctypes.data_as(_P), s_total))
-        del staging
         self.jobs = (Job * len(items))()
         for i, it in enumerate(items):
             self.jobs[i] = Job(self.src.ptr + so[i], self.dst.ptr + do[i], it["width"], it["height"], src_kind,
                                it["dxt"], it["cd"], it["nrandom"], it["refine"], it["seed"])
+
+    @classmethod
+    def from_textures(cls, textures, src_kind=SRC_RGBA8):
+        """Textures with their full mip chains built on the device
+        (Image_MipReduce32, s2tc_amd_mip_chain): only the base levels are
+        uploaded.  textures: dicts with rgba (base level), width, height, dxt,
+        cd, nrandom, refine, seed; level k of a texture encodes with seed
+        s2tc_prng_level_seed(seed, k).  items / jobs list every level, texture
+        by texture, base level first."""
+        from .workloads import level_seed
+        self = cls.__new__(cls)
+        textures = list(textures)
+        chains, s_total = [], 0
+        for t in textures:
+            layout, total = mip_layout(t["width"], t["height"])
+            chains.append((s_total, layout))
+            s_total += (total + cls.ALIGN - 1) // cls.ALIGN * cls.ALIGN
+        items, so = [], []
+        for t, (base, layout) in zip(textures, chains):
+            for k, (off, w, h) in enumerate(layout):
+                items.append(dict(rgba=None, width=w, height=h, dxt=t["dxt"], cd=t["cd"], nrandom=t["nrandom"],
+                                  refine=t["refine"], seed=level_seed(t["seed"], k)))
+                so.append(base + off)
+        self._layout(items, so, s_total, src_kind)
+        staging = np.zeros(s_total, np.uint8)
+        for t, (base, _) in zip(textures, chains):
+            a = np.ascontiguousarray(t["rgba"], dtype=np.uint8).reshape(-1)[:t["width"] * t["height"] * 4]
+            staging[base:base + a.size] = a
+        check(lib().s2tc_amd_memcpy_h2d(self.src.ptr, staging.ctypes.data_as(_P), s_total))
+        del staging
+        for t, (base, layout) in zip(textures, chains):
+            d = self.src.ptr + base
+            mip_chain_device(d, t["width"], t["height"], d, layout[-1][0] + cls.ALIGN)
+        synchronize()
+        self.uploaded_bytes = sum(t["width"] * t["height"] * 4 for t in textures)
+        return self
+
+    def source(self, i):
+        """Host copy of image i's RGBA8 source as it lies on the device."""
+        it = self.items[i]
+        n = it["width"] * it["height"] * 4
+        out = np.empty(n, np.uint8)
+        check(lib().s2tc_amd_memcpy_d2h(out.ctypes.data_as(_P), self.src.ptr + self.src_off[i], n))
+        return out
 
     def output(self, i, host=None):
         """Blocks of image i (copies the whole output buffer once if host is None)."""

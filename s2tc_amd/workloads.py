@@ -89,5 +89,14 @@ class BatchJobs:
                 yield ti, li, t, lv
 
 
+def batch_textures(seed=BATCH_SEED, count=BATCH_COUNT, serve_mix=False, nthreads=0):
+    """The base levels only, for DeviceImages.from_textures (mips built on the device)."""
+    out = []
+    for s in synth.batch_specs(count, seed, serve_mix):
+        out.append(dict(rgba=synth.texture(s, nthreads), width=s.width, height=s.height, dxt=s.dxt, cd=s.cd,
+                        nrandom=s.nrandom, refine=s.refine, seed=s.seed))
+    return out
+
+
 def batch_jobs(seed=BATCH_SEED, count=BATCH_COUNT, serve_mix=False, nthreads=0):
     return BatchJobs(synth.batch_specs(count, seed, serve_mix), nthreads)

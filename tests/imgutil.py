@@ -70,3 +70,30 @@ def smooth_rgba8(rng, w, h):
         v = 128 + 100 * np.sin(x / 7.0 + ph[c]) * np.cos(y / 9.0 + ph[c]) + rng.integers(-20, 21, (h, w))
         img[..., c] = np.clip(v, 0, 255).astype(np.uint8)
     return img.reshape(-1)
+
+
+def write_tga32(path, rgba, w, h):
+    """Uncompressed 32-bit true-colour TGA (type 2, top-left origin, BGRA order)."""
+    px = np.ascontiguousarray(rgba, dtype=np.uint8).reshape(h, w, 4)
+    hdr = bytearray(18)
+    hdr[2] = 2
+    hdr[12:14] = w.to_bytes(2, "little")
+    hdr[14:16] = h.to_bytes(2, "little")
+    hdr[16] = 32
+    hdr[17] = 0x28
+    with open(path, "wb") as f:
+        f.write(bytes(hdr))
+        f.write(px[..., [2, 1, 0, 3]].tobytes())
+
+
+def read_capture(path):
+    """[(w, h, srccomps, bytes)] recorded by oracle/_ref/libtxc_capture.so."""
+    data = open(path, "rb").read()
+    out, pos = [], 0
+    while pos < len(data):
+        w, h, c = np.frombuffer(data, np.int32, 3, pos)
+        pos += 12
+        n = int(w) * int(h) * int(c)
+        out.append((int(w), int(h), int(c), np.frombuffer(data, np.uint8, n, pos).copy()))
+        pos += n
+    return out

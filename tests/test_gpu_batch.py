@@ -64,18 +64,51 @@ def test_batch_equals_oracle_and_plain(gpu, oracle):
     imgs.free()
 
 
+def test_device_mip_sources_match_host_chain(gpu, oracle):
+    """DeviceImages.from_textures: every level built on the device equals the
+    oracle's Image_MipReduce32 chain of the base level, seeds follow
+    s2tc_prng_level_seed, and the encoded levels match the oracle."""
+    from s2tc_amd import synth, workloads
+    specs = [s for s in synth.batch_specs(80, 12, True) if s.width * s.height <= 256 * 256][:24]
+    tex = [dict(rgba=synth.texture(s), width=s.width, height=s.height, dxt=s.dxt, cd=s.cd, nrandom=s.nrandom,
+                refine=s.refine, seed=s.seed) for s in specs]
+    rng = np.random.default_rng(3)
+    for w, h in [(5, 3), (1, 9), (13, 1), (1, 1)]:
+        tex.append(dict(rgba=rng.integers(0, 256, w * h * 4).astype(np.uint8), width=w, height=h, dxt=2, cd=5,
+                        nrandom=-1, refine=1, seed=99))
+    _say(f"device mips for {len(tex)} textures")
+    imgs = gpu.DeviceImages.from_textures(tex)
+    b = gpu.Batch(imgs)
+    b.launch()
+    gpu.synchronize()
+    host = imgs.dst.to_host()
+    i = 0
+    for t in tex:
+        for k, (w, h, lv) in enumerate(oracle.mip_chain(t["rgba"], t["width"], t["height"])):
+            it = imgs.items[i]
+            assert (it["width"], it["height"]) == (w, h) and it["seed"] == workloads.level_seed(t["seed"], k)
+            assert np.array_equal(imgs.source(i), lv), (t["width"], t["height"], k)
+            q = oracle.rgb565_image(lv, w, h, 4, {0: 1, 1: 4, 2: 8}[t["dxt"]], 0)
+            exp = oracle.encode_quantized(q, w, h, t["dxt"], t["cd"], t["nrandom"], t["refine"], it["seed"])
+            assert np.array_equal(imgs.output(i, host), exp), (t["width"], t["height"], k)
+            i += 1
+    assert i == len(imgs.items)
+    b.destroy()
+    imgs.free()
+
+
 def test_full_batch_matches_reference_digest(gpu):
-    """The 2000-texture batch with all mip levels, one graph launch, against
-    the compiled reference's per-job digests."""
+    """The 2000-texture batch, one graph launch, against the compiled
+    reference's per-job digests.  Only the base levels are uploaded: the mip
+    chains are built on the device (s2tc_amd_mip_chain), while the reference
+    encoded host-built levels, so this also checks the device mip chain over
+    every level of the batch."""
     from s2tc_amd import workloads
     exp = DIGESTS["workloads"]["batch"]
-    _say("building the 2000-texture batch")
-    jobs = workloads.batch_jobs(workloads.BATCH_SEED, workloads.BATCH_COUNT)
-    items = [dict(rgba=lv["rgba"], width=lv["width"], height=lv["height"], dxt=t["dxt"], cd=t["cd"],
-                  nrandom=t["nrandom"], refine=t["refine"], seed=lv["seed"]) for _, _, t, lv in jobs.jobs()]
-    assert len(items) == exp["jobs"] and jobs.texels == exp["texels"]
-    imgs = gpu.DeviceImages(items)
-    del jobs
+    _say("building the 2000-texture batch (base levels, device mips)")
+    imgs = gpu.DeviceImages.from_textures(workloads.batch_textures(workloads.BATCH_SEED, workloads.BATCH_COUNT))
+    items = imgs.items
+    assert len(items) == exp["jobs"] and imgs.texels == exp["texels"]
     b = gpu.Batch(imgs)
     _say("full batch launch")
     b.launch()
