@@ -184,6 +184,53 @@ def quick_leg(s, size=16384, reps=10):
     return out
 
 
+def side_leg(s, reps=10):
+    """The kernels beside the encoder, each HBM-bound, timed with HIP events:
+    one mip halving of a 16384x16384 RGBA8 image (k_mip_reduce: 1 GiB read,
+    256 MiB written) and the S3TC -> S2TC remap of 2^24 DXT1 and DXT5 blocks
+    (k_from_s3tc: every block read and written once)."""
+    import numpy as np
+    ev = HipEvents()
+    out = []
+    rng = np.random.default_rng(9)
+    n = 16384
+    d_src = s.DeviceBuffer.from_host(rng.integers(0, 256, n * n * 4, dtype=np.uint8))
+    d_dst = s.DeviceBuffer(n * n)
+    s.mip_reduce_device(d_src, n, n, d_dst)
+    s.synchronize()
+    ev.record(0)
+    for _ in range(reps):
+        s.mip_reduce_device(d_src, n, n, d_dst)
+    ev.record(1)
+    ms = ev.elapsed_ms() / reps
+    nbytes = n * n * 4 + n * n
+    out.append(dict(kernel="k_mip_reduce", image=f"{n}x{n} -> {n // 2}x{n // 2}", kernel_ms=round(ms, 4),
+                    algorithmic_bytes=nbytes, achieved_gbs=round(nbytes / (ms * 1e-3) / 1e9, 1),
+                    frac=round(nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)))
+    log(f"side {out[-1]}")
+    d_dst.free()
+    for dxt in (0, 2):
+        nb = 1 << 24
+        bs = s.block_bytes(dxt)
+        d_out = s.DeviceBuffer(nb * bs)
+        s.from_s3tc_device(d_src, d_out, nb, dxt)
+        s.synchronize()
+        ev.record(0)
+        for _ in range(reps):
+            s.from_s3tc_device(d_src, d_out, nb, dxt)
+        ev.record(1)
+        ms = ev.elapsed_ms() / reps
+        nbytes = 2 * nb * bs
+        out.append(dict(kernel=f"k_from_s3tc<{dxt}>", blocks=nb, dxt=(1, 3, 5)[dxt], kernel_ms=round(ms, 4),
+                        algorithmic_bytes=nbytes, achieved_gbs=round(nbytes / (ms * 1e-3) / 1e9, 1),
+                        frac=round(nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)))
+        log(f"side {out[-1]}")
+        d_out.free()
+    d_src.free()
+    ev.close()
+    return out
+
+
 def quality_leg(s, w=1024, h=512):
     """Total colour distance (s2tc_amd_colordist_error) of a NORMALMAP,
     nrandom 64, REFINE_LOOP, DXT1 encoding of a w x h normal map (512
@@ -267,6 +314,9 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-quick", action="store_true")
     ap.add_argument("--quick-size", type=int, default=16384)
+    ap.add_argument("--no-side", action="store_true")
+    ap.add_argument("--only-side", action="store_true",
+                    help="only run the mip / transcode kernels and print their JSON")
     ap.add_argument("--only-quick", action="store_true",
                     help="only run the quick-mode leg (one large image) and print its JSON")
     ap.add_argument("--serve-duration", type=float, default=0.5)
@@ -290,6 +340,10 @@ def main():
             s.check(L.s2tc_amd_memcpy_d2d_kernel(dst.ptr, src.ptr, n, None))
         s.synchronize()
         print(json.dumps({"calibrate_copy_bytes": n}))
+        return
+
+    if a.only_side:
+        print(json.dumps({"side_kernels": side_leg(s, max(1, a.steps))}), flush=True)
         return
 
     if a.only_quick:
@@ -383,6 +437,10 @@ def main():
     if not a.no_quick and a.workload == "batch":
         quick = quick_leg(s, a.quick_size)
 
+    side = None
+    if not a.no_side and a.workload == "batch":
+        side = side_leg(s)
+
     serve = None
     if not a.no_serve and a.workload == "batch":
         serve = serve_leg(s, (2000, 10000, 50000), a.serve_duration)
@@ -410,6 +468,7 @@ def main():
         "roofline": roofline,
         "classes": class_summary,
         "quick_mode": quick,
+        "side_kernels": side,
         "cpu_baseline": cpu,
         "serve": serve,
         "quality": quality,

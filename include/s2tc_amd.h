@@ -153,6 +153,13 @@ S2TC_API int s2tc_amd_mip_reduce(const void *d_in, int width, int height, void *
 S2TC_API int s2tc_amd_mip_chain(const void *d_src, int width, int height, void *d_levels, size_t capacity,
                                 void *stream);
 
+/* s2tc_from_s3tc on the device: remaps nblocks S3TC blocks to S2TC
+ * (s2tc_from_s3tc.cpp:77-190 per block, dispatched as :254-263).  dxt: 0
+ * DXT1 (8-byte blocks, transparent mode kept), 1 DXT3, 2 DXT5 (16-byte
+ * blocks).  d_in may equal d_out; buffers 8- (DXT1) or 16-byte aligned.
+ * Asynchronous on `stream`. */
+S2TC_API int s2tc_amd_from_s3tc(const void *d_in, void *d_out, size_t nblocks, int dxt, void *stream);
+
 /* The same jobs with one plain kernel launch per job (the per-texture path). */
 S2TC_API int s2tc_amd_encode_jobs_plain(const s2tc_amd_job *jobs, int njobs, void *stream);
 

@@ -59,6 +59,7 @@ class Oracle(_Base):
         L.oracle_srgb_get_y.argtypes = [I, I, I]
         L.oracle_colordist_error.argtypes = [P, I, I, I, I, P]
         L.oracle_colordist_error.restype = ctypes.c_int64
+        L.oracle_from_s3tc_block.argtypes = [P, I]
         self._encode = L.oracle_encode_image_quantized
 
     def rgb565_image(self, rgba, width, height, srccomps, alphabits, dither):
@@ -100,6 +101,17 @@ class Oracle(_Base):
         q = np.ascontiguousarray(q, dtype=np.uint8)
         blocks = np.ascontiguousarray(blocks, dtype=np.uint8)
         return int(self.lib.oracle_colordist_error(_ptr(q), width, height, dxt, cd, _ptr(blocks)))
+
+
+    def from_s3tc(self, blocks, dxt):
+        """s2tc_from_s3tc's remap of every block of a DXT1 (dxt 0), DXT3 (1) or DXT5 (2) block array."""
+        bs = block_bytes(dxt)
+        out = np.array(blocks, dtype=np.uint8, copy=True).reshape(-1)
+        n = out.size // bs
+        base = out.ctypes.data
+        for i in range(n):
+            self.lib.oracle_from_s3tc_block(ctypes.c_void_p(base + i * bs), dxt)
+        return out
 
 
 class Reference(_Base):
@@ -148,6 +160,12 @@ class StockReference(_Base):
 
     def srand(self, seed=1):
         self.lib.ref_stock_srand(seed)
+
+
+def reference_from_s3tc_path():
+    """The reference's own s2tc_from_s3tc built from its source, or None."""
+    p = os.path.join(HERE, "_ref", "s2tc_from_s3tc")
+    return p if os.path.exists(p) else None
 
 
 def stock_reference_available():

@@ -840,3 +840,72 @@ void oracle_mip_reduce(const uint8_t *in, uint8_t *out, int *width, int *height)
 					out[c] = (uint8_t)((in[c] + in[nextrow + c]) >> 1);
 	}
 }
+
+/* ---- s2tc_from_s3tc: s2tc_from_s3tc.cpp:77-190, 254-263 ----
+ *
+ * Restated per texel index rather than with the reference's word-wide bit
+ * operations.  Texel k = 4y + x of the block; "checker" is (x + y) & 1, the
+ * pattern in which the reference sends an index that S2TC cannot express to
+ * one of the two endpoints.
+ *
+ * Colour block (2-bit indices, words c0 = bytes 0-1, c1 = bytes 2-3):
+ *   keep3 (c1 >= c0 in a DXT1 block: three colours + transparent, kept):
+ *     0 -> 0, 1 -> 1, 2 -> checker, 3 -> 3;
+ *   no3   (every other case): 0 -> 0, 1 -> 1, 2 -> checker, 3 -> checker;
+ *   then, to give S2TC's endpoint order, the endpoints swap and indices
+ *   0 <-> 1 flip: in a DXT1 block when c1 < c0, in a DXT3/5 colour block
+ *   when c1 >= c0.
+ * DXT5 alpha block (3-bit indices, a0 = byte 0, a1 = byte 1):
+ *   a1 >= a0 (six-value mode): 0, 1, 6, 7 kept, 2..5 -> checker;
+ *   a1 <  a0 (eight-value mode): 0, 1 kept, 2..7 -> checker, then the
+ *   endpoints swap and indices 0 <-> 1 flip. */
+static int fs_checker(int k) { return ((k & 3) + (k >> 2)) & 1; }
+
+static void fs_colour(uint8_t *b, int dxt1)
+{
+	const unsigned c0 = b[0] | (b[1] << 8), c1 = b[2] | (b[3] << 8);
+	const uint32_t idx = b[4] | (b[5] << 8) | (b[6] << 16) | ((uint32_t)b[7] << 24);
+	const int keep3 = dxt1 && c1 >= c0;
+	const int swap = dxt1 ? c1 < c0 : c1 >= c0;
+	uint32_t out = 0;
+	for (int k = 0; k < 16; ++k) {
+		int v = (idx >> (2 * k)) & 3;
+		if (v == 2 || (v == 3 && !keep3)) v = fs_checker(k);
+		if (swap) v ^= 1;   /* v is 0 or 1 here: a swap never meets a kept 3 */
+		out |= (uint32_t)v << (2 * k);
+	}
+	if (swap) {
+		b[0] = (uint8_t)c1; b[1] = (uint8_t)(c1 >> 8);
+		b[2] = (uint8_t)c0; b[3] = (uint8_t)(c0 >> 8);
+	}
+	for (int i = 0; i < 4; ++i) b[4 + i] = (uint8_t)(out >> (8 * i));
+}
+
+static void fs_alpha(uint8_t *b)
+{
+	const int a0 = b[0], a1 = b[1];
+	uint64_t idx = 0, out = 0;
+	for (int i = 0; i < 6; ++i) idx |= (uint64_t)b[2 + i] << (8 * i);
+	const int six = a1 >= a0;
+	for (int k = 0; k < 16; ++k) {
+		int v = (int)((idx >> (3 * k)) & 7);
+		if (six ? (v >= 2 && v <= 5) : v >= 2) v = fs_checker(k);
+		if (!six) v ^= 1;
+		out |= (uint64_t)v << (3 * k);
+	}
+	if (!six) {
+		b[0] = (uint8_t)a1;
+		b[1] = (uint8_t)a0;
+	}
+	for (int i = 0; i < 6; ++i) b[2 + i] = (uint8_t)(out >> (8 * i));
+}
+
+void oracle_from_s3tc_block(uint8_t *block, int dxt)
+{
+	if (dxt == O_DXT1) {
+		fs_colour(block, 1);
+		return;
+	}
+	fs_colour(block + 8, 0);
+	if (dxt == O_DXT5) fs_alpha(block);
+}

@@ -72,6 +72,12 @@ int64_t oracle_colordist_error(const uint8_t *q, int width, int height, int dxt,
 /* Image_MipReduce32 to (destwidth, destheight) = (1, 1); in may equal out. */
 void oracle_mip_reduce(const uint8_t *in, uint8_t *out, int *width, int *height);
 
+/* s2tc_from_s3tc's per-block remap of an S3TC block to S2TC, in place:
+ * dxt O_DXT1 -> convert_dxt1a on the 8-byte block, O_DXT3 -> convert_dxt1 on
+ * bytes 8-15, O_DXT5 -> convert_dxt1 on bytes 8-15 and convert_dxt5 on bytes
+ * 0-7 (s2tc_from_s3tc.cpp:77-190, 254-263). */
+void oracle_from_s3tc_block(uint8_t *block, int dxt);
+
 #ifdef __cplusplus
 }
 #endif

@@ -56,6 +56,7 @@ def lib():
                                       ctypes.POINTER(_I), _I, ctypes.POINTER(_SZ)]
     L.s2tc_amd_mip_reduce.argtypes = [_P, _I, _I, _P, _P]
     L.s2tc_amd_mip_chain.argtypes = [_P, _I, _I, _P, _SZ, _P]
+    L.s2tc_amd_from_s3tc.argtypes = [_P, _P, _SZ, _I, _P]
     _lib = L
     return L
 
@@ -219,6 +220,28 @@ def mip_chain(rgba, width, height):
         d_s.free()
         d_l.free()
     return [(w, h, buf[o:o + w * h * 4].copy()) for o, w, h in layout]
+
+
+# ---- S3TC -> S2TC transcoding ----
+
+def from_s3tc_device(d_in, d_out, nblocks, dxt, stream=None):
+    """s2tc_from_s3tc's block remap of nblocks blocks on the device (asynchronous; in place allowed)."""
+    check(lib().s2tc_amd_from_s3tc(_dptr(d_in), _dptr(d_out), nblocks, dxt, stream))
+
+
+def from_s3tc(blocks, dxt):
+    """Host convenience: remapped copy of a DXT1 (dxt 0), DXT3 (1) or DXT5 (2) block array."""
+    blocks = np.ascontiguousarray(blocks, dtype=np.uint8).reshape(-1)
+    n = blocks.size // block_bytes(dxt)
+    if n == 0:
+        return blocks[:0].copy()
+    d = DeviceBuffer.from_host(blocks[:n * block_bytes(dxt)])
+    try:
+        from_s3tc_device(d, d, n, dxt)
+        synchronize()
+        return d.to_host(n * block_bytes(dxt))
+    finally:
+        d.free()
 
 
 # ---- batches ----

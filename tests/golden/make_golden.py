@@ -14,6 +14,8 @@ Files written:
   compress.npz  tx_compress_dxtn outputs (environment settings, row strides)
   digests.json  SHA-256 of full-size workload outputs and of the reference
                 CLI's DDS files for the test images
+  from_s3tc.npz DDS files of random S3TC blocks and the reference
+                s2tc_from_s3tc's output for each (oracle/_ref/s2tc_from_s3tc)
 """
 import argparse
 import hashlib
@@ -189,6 +191,37 @@ def batch_digest_reference(ref, jobs):
     return {"jobs": len(hashes), "texels": jobs.texels, "sha256_of_job_sha256s": hashlib.sha256(b"".join(hashes)).hexdigest()}
 
 
+def s3tc_dds(rng, fourcc, nblocks, bs, tail=3):
+    """A DDS file: 128-byte header carrying fourcc, random blocks (some with
+    equal endpoints), tail bytes of a partial block."""
+    blocks = rng.integers(0, 256, (nblocks, bs)).astype(np.uint8)
+    blocks[::7, bs - 6:bs - 4] = blocks[::7, bs - 8:bs - 6]      # c1 == c0
+    if bs == 16:
+        blocks[::5, 1] = blocks[::5, 0]                          # a1 == a0
+    hdr = np.zeros(128, np.uint8)
+    hdr[0:4] = np.frombuffer(b"DDS ", np.uint8)
+    hdr[4:128] = rng.integers(0, 256, 124)
+    hdr[84:88] = np.frombuffer(fourcc, np.uint8)
+    return np.concatenate([hdr, blocks.reshape(-1), rng.integers(0, 256, tail).astype(np.uint8)])
+
+
+def from_s3tc_corpus():
+    exe = os.path.join(ROOT, "oracle", "_ref", "s2tc_from_s3tc")
+    rng = np.random.default_rng(777)
+    out = {}
+    for fourcc, bs in ((b"DXT1", 8), (b"DXT3", 16), (b"DXT5", 16)):
+        dds = s3tc_dds(rng, fourcc, 2048, bs)
+        with tempfile.TemporaryDirectory() as td:
+            i, o = os.path.join(td, "i.dds"), os.path.join(td, "o.dds")
+            open(i, "wb").write(dds.tobytes())
+            subprocess.run([exe, "-i", i, "-o", o], check=True, capture_output=True)
+            res = np.frombuffer(open(o, "rb").read(), np.uint8)
+        out[f"{fourcc.decode()}_in"] = dds
+        out[f"{fourcc.decode()}_out"] = res
+    np.savez_compressed(os.path.join(HERE, "from_s3tc.npz"), **out)
+    return len(out) // 2
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--skip-large", action="store_true")
@@ -206,6 +239,8 @@ def main():
         print("compress cases", compress_corpus(ref))
     if not only or "cli" in only:
         digests["cli"] = cli_digests()
+    if not only or "from_s3tc" in only:
+        print("from_s3tc cases", from_s3tc_corpus())
     if not only or "workloads" in only:
         digests.setdefault("workloads", {}).update(workload_digests(ref, a.skip_large))
     json.dump(digests, open(dig_path, "w"), indent=1, sort_keys=True)
