@@ -46,12 +46,15 @@ S2TC_D int dpair(const int (&D)[120], int i, int k) { return k == i ? 0 : D[k < 
 template <bool F>
 S2TC_D uint32_t fold_pairs_key(const int (&D)[120], const int (&f)[16], uint32_t use)
 {
+	uint32_t pen[16];
+#pragma unroll
+	for (int k = 0; k < 16; ++k) pen[k] = ((use >> k) & 1u) ? 0u : (1u << 23);
 	uint32_t best = 0xffffffffu;
 #pragma unroll
 	for (int i = 0; i < 16; ++i) {
 #pragma unroll
 		for (int j = i + 1; j < 16; ++j) {
-			uint32_t s = ((use >> i) & (use >> j) & 1u) ? 0u : (1u << 23);
+			uint32_t s = pen[i] | pen[j];
 #pragma unroll
 			for (int k = 0; k < 16; ++k) {
 				if (k == i || k == j) continue;
@@ -107,6 +110,8 @@ S2TC_D BlockOut encode_block_all16(const uint32_t (&tx)[16], int w, int h, int r
 		use |= ((v && !(DXT == DXT1 && texel_a(tx[k]) == 0)) ? 1u : 0u) << k;
 	}
 	const int n = __builtin_popcount(use);
+	// a full DXT5 block with one alpha needs no alpha search
+	const bool auni = DXT == DXT5 && valid == 0xffffu && alpha_uniform(tx);
 
 	uint32_t c0, c1;
 	int a0 = 0, a1 = 0;
@@ -132,14 +137,20 @@ S2TC_D BlockOut encode_block_all16(const uint32_t (&tx)[16], int w, int h, int r
 			// candidate set contribute nothing to any sum
 			int D[120];
 			{
+				// every distance is computed (no per-pair branch) and then
+				// cleared where a slot is not a candidate
 				F f[16];
+				uint32_t um[16];
 #pragma unroll
-				for (int k = 0; k < 16; ++k) f[k] = Dd::featq(tx[k]);
+				for (int k = 0; k < 16; ++k) {
+					f[k] = Dd::featq(tx[k]);
+					um[k] = 0u - ((use >> k) & 1u);
+				}
 #pragma unroll
 				for (int i = 0; i < 16; ++i)
 #pragma unroll
 					for (int j = i + 1; j < 16; ++j)
-						D[tri(i, j)] = (((use >> i) & (use >> j)) & 1u) ? Dd::d(f[i], f[j]) : 0;
+						D[tri(i, j)] = (int)((uint32_t)Dd::d(f[i], f[j]) & um[i] & um[j]);
 			}
 			__builtin_amdgcn_sched_barrier(0);
 			uint32_t p;
@@ -160,7 +171,7 @@ S2TC_D BlockOut encode_block_all16(const uint32_t (&tx)[16], int w, int h, int r
 		}
 		c0 = v0;
 		c1 = v1;
-		if (DXT == DXT5) {
+		if (DXT == DXT5 && !auni) {
 			// keep the alpha triangle out of the colour triangle's live range
 			__builtin_amdgcn_sched_barrier(0);
 			int ai, aj;
@@ -213,10 +224,15 @@ S2TC_D BlockOut encode_block_all16(const uint32_t (&tx)[16], int w, int h, int r
 		o.w[2] = c0 | (c1 << 16);
 		o.w[3] = bits;
 	} else {
-		a1 = fix_equal_a(a0, a1);
 		const uint32_t bits = color_refine_lane<CD, false, false>(tx, f, mk, refine, c0, c1);
-		const uint64_t ab = alpha_refine_fast<false>(tx, mk.valid, refine, a0, a1);
-		const uint64_t lo = (uint64_t)(a0 & 255) | ((uint64_t)(a1 & 255) << 8) | (ab << 16);
+		uint64_t lo;
+		if (auni) {
+			lo = alpha_uniform_half(texel_a(tx[0]));
+		} else {
+			a1 = fix_equal_a(a0, a1);
+			const uint64_t ab = alpha_refine_fast<false>(tx, mk.valid, refine, a0, a1);
+			lo = (uint64_t)(a0 & 255) | ((uint64_t)(a1 & 255) << 8) | (ab << 16);
+		}
 		o.w[0] = (uint32_t)lo;
 		o.w[1] = (uint32_t)(lo >> 32);
 		o.w[2] = c0 | (c1 << 16);

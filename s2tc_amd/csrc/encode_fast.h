@@ -369,10 +369,15 @@ S2TC_D BlockOut encode_block_fast_t(const uint32_t (&tx)[16], int w, int h, int 
 		o.w[2] = c0 | (c1 << 16);
 		o.w[3] = bits;
 	} else {
-		a1 = fix_equal_a(a0, a1);
 		const uint32_t bits = color_refine_lane<CD, false, FULL>(tx, f, mk, refine, c0, c1);
-		const uint64_t ab = alpha_refine_fast<FULL>(tx, mk.valid, refine, a0, a1);
-		const uint64_t lo = (uint64_t)(a0 & 255) | ((uint64_t)(a1 & 255) << 8) | (ab << 16);
+		uint64_t lo;
+		if (FULL && alpha_uniform(tx)) {
+			lo = alpha_uniform_half(texel_a(tx[0]));
+		} else {
+			a1 = fix_equal_a(a0, a1);
+			const uint64_t ab = alpha_refine_fast<FULL>(tx, mk.valid, refine, a0, a1);
+			lo = (uint64_t)(a0 & 255) | ((uint64_t)(a1 & 255) << 8) | (ab << 16);
+		}
 		o.w[0] = (uint32_t)lo;
 		o.w[1] = (uint32_t)(lo >> 32);
 		o.w[2] = c0 | (c1 << 16);

@@ -323,17 +323,22 @@ S2TC_D BlockOut encode_block_quad(const uint32_t (&px)[16], const uint32_t *__re
 		o.w[2] = c0 | (c1 << 16);
 		o.w[3] = bits;
 	} else {
-		a1 = fix_equal_a(a0, a1);
 		const uint32_t bits = quad_color_refine<CD, false>(U, P, ntot, 0u, Q, refine, c0, c1);
 		// the alpha byte is the same in RAW and quantized DXT5 words
-		uint32_t valid = 0xffffu;
-		if (!FULL) {
-			valid = 0;
+		uint64_t lo;
+		if (FULL && alpha_uniform(px)) {
+			lo = alpha_uniform_half(texel_a(px[0]));
+		} else {
+			a1 = fix_equal_a(a0, a1);
+			uint32_t valid = 0xffffu;
+			if (!FULL) {
+				valid = 0;
 #pragma unroll
-			for (int k = 0; k < 16; ++k) valid |= ((k >> 2) < w && (k & 3) < h ? 1u : 0u) << k;
+				for (int k = 0; k < 16; ++k) valid |= ((k >> 2) < w && (k & 3) < h ? 1u : 0u) << k;
+			}
+			const uint64_t ab = alpha_refine_fast<FULL>(px, valid, refine, a0, a1);
+			lo = (uint64_t)(a0 & 255) | ((uint64_t)(a1 & 255) << 8) | (ab << 16);
 		}
-		const uint64_t ab = alpha_refine_fast<FULL>(px, valid, refine, a0, a1);
-		const uint64_t lo = (uint64_t)(a0 & 255) | ((uint64_t)(a1 & 255) << 8) | (ab << 16);
 		o.w[0] = (uint32_t)lo;
 		o.w[1] = (uint32_t)(lo >> 32);
 		o.w[2] = c0 | (c1 << 16);

@@ -286,4 +286,32 @@ S2TC_D int fix_equal_a(int a0, int a1)
 	return a0 == 255 ? a1 - 1 : a1 + 1;
 }
 
+// True if the 16 words share their top byte (the alpha of a DXT5 texel, in
+// quantized and in RGBA8 words alike).
+S2TC_D bool alpha_uniform(const uint32_t (&tx)[16])
+{
+	uint32_t d = 0;
+#pragma unroll
+	for (int k = 1; k < 16; ++k) d |= tx[k] ^ tx[0];
+	return (d >> 24) == 0;
+}
+
+// The DXT5 alpha half (bytes 0-7 as one word) of a full block whose 16
+// alphas all equal a.  Every endpoint search yields (a, a): the quick
+// minimum and maximum agree, and in the pair scans every pair sums to 0 so
+// the first pair wins.  fix_equal_a (:1009-1016) makes that (a, a + 1), or
+// (255, 254) for a = 255, and one pass gives every texel the same index: 0
+// (the exact endpoint) for 1 <= a <= 254, the fixpoint 6 (alpha 0) for
+// a = 0, the fixpoint 7 (alpha 255) for a = 255, where the order fix-up
+// (:673-705) swaps the endpoints to (254, 255).  No refinement moves an
+// endpoint (the texels at endpoint 0 average to a, none is at endpoint 1,
+// fixpoint texels count for neither), so NEVER, ALWAYS and LOOP agree.
+S2TC_D uint64_t alpha_uniform_half(int a)
+{
+	const uint64_t all6 = 0xDB6DB6DB6DB6ull, all7 = 0xFFFFFFFFFFFFull;
+	if (a == 0) return 0x0100ull | (all6 << 16);
+	if (a == 255) return 0xFFFEull | (all7 << 16);
+	return (uint64_t)a | ((uint64_t)(a + 1) << 8);
+}
+
 }  // namespace s2tc

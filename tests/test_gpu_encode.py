@@ -105,3 +105,31 @@ def test_quick_quadratic_modes(gpu, oracle, cd):
                 assert np.array_equal(got, exp), f"raw {kind} {w}x{h} dxt={dxt} cd={cd} refine={refine}"
                 got = gpu.encode_image(q, w, h, dxt, cd, -1, refine, 0)
                 assert np.array_equal(got, exp), f"quantized {kind} {w}x{h} dxt={dxt} cd={cd} refine={refine}"
+
+
+@pytest.mark.parametrize("raw", [False, True])
+def test_uniform_alpha_blocks(gpu, oracle, raw):
+    """DXT5 blocks whose alphas are all equal take a closed form for the
+    alpha half: every alpha 0..255 in full blocks, next to blocks that differ
+    in one texel's alpha and to partial blocks in the same waves, in quick
+    mode (quadratic and generic distances) and with all 16 candidates."""
+    rng = np.random.default_rng(55)
+    w, h = 4 * 256 + 3, 4 * 3 + 2
+    rgba = rng.integers(0, 256, (h, w, 4)).astype(np.uint8)
+    alpha = np.repeat(np.arange(256, dtype=np.uint8), 4)
+    for r in range(h):
+        rgba[r, :1024, 3] = alpha
+    rgba[4:8, 5::9, 3] ^= 1          # one texel off in some blocks of the second block row
+    rgba[8:12, :1024:4, 3] = 255     # the third block row: many opaque blocks, a few not
+    rgba[8:12, 40::37, 3] = 17
+    for refine in range(3):
+        for cd, nrandom in [(5, -1), (4, -1), (2, -1), (7, -1), (3, 0), (7, 0)]:
+            _say(f"case uniform alpha raw={raw} refine={refine} cd={cd} nrandom={nrandom}")
+            if raw:
+                got = gpu.encode_image(rgba.reshape(-1), w, h, 2, cd, nrandom, refine, src_kind=gpu.SRC_RGBA8)
+                q = oracle.rgb565_image(rgba.reshape(-1), w, h, 4, 8, 0)
+            else:
+                q = oracle.rgb565_image(rgba.reshape(-1), w, h, 4, 8, 0)
+                got = gpu.encode_image(q, w, h, 2, cd, nrandom, refine)
+            exp = oracle.encode_quantized(q, w, h, 2, cd, nrandom, refine, 0)
+            assert np.array_equal(got, exp), (raw, refine, cd, nrandom)

@@ -116,3 +116,31 @@ def test_pair_key_packing_fits():
     for bound in (400100, 11657, 5891, 20681, 16256):  # NORMALMAP, AVG, W0AVG, WAVG, alpha with fixpoints
         assert 16 * bound < (1 << 23)
         assert (16 * bound + (1 << 23)) << 8 < (1 << 32)
+
+
+def _uniform_alpha_half(a):
+    """alpha_uniform_half of s2tc_device.h: the DXT5 alpha bytes of a full block with one alpha."""
+    all6, all7 = sum(6 << (3 * k) for k in range(16)), (1 << 48) - 1
+    if a == 0:
+        return 0x0100 | (all6 << 16)
+    if a == 255:
+        return 0xFFFE | (all7 << 16)
+    return a | ((a + 1) << 8)
+
+
+def test_uniform_alpha_closed_form(oracle):
+    """A full DXT5 block whose 16 alphas are equal encodes its alpha half as
+    alpha_uniform_half says, for every alpha, refinement, quick and
+    all-candidate modes and distances (the oracle restates the reference)."""
+    rng = np.random.default_rng(12)
+    for refine in range(3):
+        for cd, nrandom in [(5, -1), (2, -1), (3, -1), (7, -1), (0, 0), (7, 0), (4, 0)]:
+            q = np.zeros((4, 4 * 256, 4), np.uint8)
+            q[..., 0] = rng.integers(0, 32, (4, 1024))
+            q[..., 1] = rng.integers(0, 64, (4, 1024))
+            q[..., 2] = rng.integers(0, 32, (4, 1024))
+            q[..., 3] = np.repeat(np.arange(256, dtype=np.uint8), 4)[None, :]
+            out = oracle.encode_quantized(q.reshape(-1), 1024, 4, 2, cd, nrandom, refine, 0).reshape(-1, 16)
+            for a in range(256):
+                got = int.from_bytes(out[a, :8].tobytes(), "little")
+                assert got == _uniform_alpha_half(a), (refine, cd, nrandom, a)
