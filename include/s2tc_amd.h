@@ -153,6 +153,15 @@ S2TC_API int s2tc_amd_mip_reduce(const void *d_in, int width, int height, void *
 S2TC_API int s2tc_amd_mip_chain(const void *d_src, int width, int height, void *d_levels, size_t capacity,
                                 void *stream);
 
+/* Many chains at once, one launch per level: each job's buffer holds level 0
+ * in place and room for every level as s2tc_amd_mip_layout places them.
+ * Returns after the chains are built (the stream is synchronized). */
+typedef struct {
+	void *levels;      /* device: the chain buffer, level 0 already filled */
+	int width, height; /* level 0 */
+} s2tc_amd_mip_job;
+S2TC_API int s2tc_amd_mip_chain_batch(const s2tc_amd_mip_job *jobs, int njobs, void *stream);
+
 /* s2tc_from_s3tc on the device: remaps nblocks S3TC blocks to S2TC
  * (s2tc_from_s3tc.cpp:77-190 per block, dispatched as :254-263).  dxt: 0
  * DXT1 (8-byte blocks, transparent mode kept), 1 DXT3, 2 DXT5 (16-byte

@@ -57,6 +57,7 @@ def lib():
     L.s2tc_amd_mip_reduce.argtypes = [_P, _I, _I, _P, _P]
     L.s2tc_amd_mip_chain.argtypes = [_P, _I, _I, _P, _SZ, _P]
     L.s2tc_amd_from_s3tc.argtypes = [_P, _P, _SZ, _I, _P]
+    L.s2tc_amd_mip_chain_batch.argtypes = [_P, _I, _P]
     _lib = L
     return L
 
@@ -207,6 +208,18 @@ def mip_chain_device(d_src, width, height, d_levels, capacity, stream=None):
     return n
 
 
+class MipJob(ctypes.Structure):
+    _fields_ = [("levels", _P), ("width", _I), ("height", _I)]
+
+
+def mip_chain_batch(jobs, stream=None):
+    """Chains of many images in place, one launch per level: jobs is a list of
+    (device address of a chain buffer with level 0 filled, width, height).
+    Returns after the chains are built."""
+    arr = (MipJob * len(jobs))(*[MipJob(p, w, h) for p, w, h in jobs])
+    check(lib().s2tc_amd_mip_chain_batch(ctypes.cast(arr, _P), len(jobs), stream))
+
+
 def mip_chain(rgba, width, height):
     """Host convenience: [(w, h, RGBA8 bytes)] of the chain computed on the GPU."""
     rgba = np.ascontiguousarray(rgba, dtype=np.uint8).reshape(-1)[:width * height * 4]
@@ -343,10 +356,7 @@ class DeviceImages:
             staging[base:base + a.size] = a
         check(lib().s2tc_amd_memcpy_h2d(self.src.ptr, staging.ctypes.data_as(_P), s_total))
         del staging
-        for t, (base, layout) in zip(textures, chains):
-            d = self.src.ptr + base
-            mip_chain_device(d, t["width"], t["height"], d, layout[-1][0] + cls.ALIGN)
-        synchronize()
+        mip_chain_batch([(self.src.ptr + base, t["width"], t["height"]) for t, (base, _) in zip(textures, chains)])
         self.uploaded_bytes = sum(t["width"] * t["height"] * 4 for t in textures)
         return self
 

@@ -341,22 +341,20 @@ __global__ __launch_bounds__(256) void k_colordist_error(const uint32_t *__restr
 	if (lane == 0) atomicAdd((unsigned long long *)sum, (unsigned long long)(part[0] + part[1] + part[2] + part[3]));
 }
 
+// Per-device tables of this kernel object's distance (one thread per RGB565
+// colour, 256 x 256 threads).
+template <int CD>
+__global__ __launch_bounds__(256) void k_tables_init(int)
+{
+	const uint32_t v = blockIdx.x * 256u + threadIdx.x;
 #ifdef S2TC_SRGB_Y_TABLE
-__global__ __launch_bounds__(256) void k_srgb_y_table_init(int)
-{
-	const uint32_t v = blockIdx.x * 256u + threadIdx.x;
 	s2tc_srgb_y_table[v] = (uint16_t)srgb_get_y(c_r(v), c_g(v), c_b(v));
-}
 #endif
-
 #ifdef S2TC_NM_TABLE
-__global__ __launch_bounds__(256) void k_nm_table_init(int)
-{
-	const uint32_t v = blockIdx.x * 256u + threadIdx.x;
 	const FeatF f = Dist<CD_NORMALMAP>::compute(v);
 	s2tc_nm_table[v] = make_float4(f.x, f.y, f.z, 0.0f);
-}
 #endif
+}
 
 }  // namespace s2tc
 
@@ -418,10 +416,8 @@ extern "C" void S2TC_CAT(s2tc_kernels_cd, S2TC_CD)(KernelTable *kt)
 #undef S2TC_AB3
 	kt->normal[0] = (const void *)k_normal<S2TC_CD, false>;
 	kt->normal[1] = (const void *)k_normal<S2TC_CD, true>;
-#if defined(S2TC_NM_TABLE)
-	kt->init = (const void *)k_nm_table_init;
-#elif defined(S2TC_SRGB_Y_TABLE)
-	kt->init = (const void *)k_srgb_y_table_init;
+#if defined(S2TC_NM_TABLE) || defined(S2TC_SRGB_Y_TABLE)
+	kt->init = (const void *)k_tables_init<S2TC_CD>;
 #else
 	kt->init = nullptr;
 #endif

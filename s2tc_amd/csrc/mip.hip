@@ -10,6 +10,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 #include "s2tc_internal.h"
 
 namespace s2tc {
@@ -28,12 +30,32 @@ struct Sum4 {
 	}
 };
 
+__device__ __forceinline__ void mip_texel(const uint32_t *__restrict__ in, int width, int height,
+                                          uint32_t *__restrict__ out, int x, int y, int ow);
+
 __global__ __launch_bounds__(256) void k_mip_reduce(const uint32_t *__restrict__ in, int width, int height,
                                                     uint32_t *__restrict__ out)
 {
 	const int ow = width > 1 ? width >> 1 : 1, oh = height > 1 ? height >> 1 : 1;
 	const int x = (int)(blockIdx.x * 64 + threadIdx.x), y = (int)(blockIdx.y * 4 + threadIdx.y);
 	if (x >= ow || y >= oh) return;
+	mip_texel(in, width, height, out, x, y, ow);
+}
+
+// One level of many chains in one launch: a task is one texture's halving
+// at this level; tasks own consecutive runs of 256-texel tiles (tile0 is the
+// first), and a workgroup finds its task by a binary search over tile0.
+struct MipTask {
+	const uint32_t *in;
+	uint32_t *out;
+	int w, h;          // input size
+	uint32_t tile0;
+	uint32_t pad;
+};
+
+__device__ __forceinline__ void mip_texel(const uint32_t *__restrict__ in, int width, int height,
+                                          uint32_t *__restrict__ out, int x, int y, int ow)
+{
 	Sum4 s{0u, 0u};
 	int shift;
 	if (width > 1 && height > 1) {
@@ -55,6 +77,23 @@ __global__ __launch_bounds__(256) void k_mip_reduce(const uint32_t *__restrict__
 		shift = 1;
 	}
 	out[(size_t)y * (size_t)ow + x] = s.shifted(shift);
+}
+
+__global__ __launch_bounds__(256) void k_mip_reduce_batch(const MipTask *__restrict__ tasks, int ntasks)
+{
+	const uint32_t tile = blockIdx.x;
+	int lo = 0, hi = ntasks - 1;
+	while (lo < hi) {
+		const int mid = (lo + hi + 1) >> 1;
+		if (tasks[mid].tile0 <= tile) lo = mid;
+		else hi = mid - 1;
+	}
+	const MipTask t = tasks[lo];
+	const int ow = t.w > 1 ? t.w >> 1 : 1, oh = t.h > 1 ? t.h >> 1 : 1;
+	const uint32_t i = (tile - t.tile0) * 256u + threadIdx.x;
+	if (i >= (uint32_t)ow * (uint32_t)oh) return;
+	const int y = (int)(i / (uint32_t)ow), x = (int)(i - (uint32_t)y * (uint32_t)ow);
+	mip_texel(t.in, t.w, t.h, t.out, x, y, ow);
 }
 
 }  // namespace s2tc
@@ -122,4 +161,65 @@ extern "C" int s2tc_amd_mip_chain(const void *d_src, int width, int height, void
 		h = h > 1 ? h >> 1 : 1;
 	}
 	return rc ? rc : n;
+}
+
+extern "C" int s2tc_amd_mip_chain_batch(const s2tc_amd_mip_job *jobs, int njobs, void *stream)
+{
+	if (njobs < 0 || (njobs && !jobs)) return set_error(S2TC_E_ARG, "mip_chain_batch: bad arguments");
+	// per texture: level offsets and sizes; then the tasks of every level
+	std::vector<std::vector<MipTask>> levels;
+	for (int j = 0; j < njobs; ++j) {
+		const s2tc_amd_mip_job &jb = jobs[j];
+		if (!jb.levels || jb.width <= 0 || jb.height <= 0)
+			return set_error(S2TC_E_ARG, "mip_chain_batch: job %d: bad image", j);
+		int w = jb.width, h = jb.height, l = 0;
+		size_t off = 0;
+		while (!(w == 1 && h == 1)) {
+			const size_t next = off + align256((size_t)w * h * 4);
+			if ((size_t)l >= levels.size()) levels.emplace_back();
+			MipTask t;
+			t.in = (const uint32_t *)((const uint8_t *)jb.levels + off);
+			t.out = (uint32_t *)((uint8_t *)jb.levels + next);
+			t.w = w;
+			t.h = h;
+			t.tile0 = 0;
+			t.pad = 0;
+			levels[l].push_back(t);
+			off = next;
+			w = w > 1 ? w >> 1 : 1;
+			h = h > 1 ? h >> 1 : 1;
+			++l;
+		}
+	}
+	if (levels.empty()) return 0;
+	std::vector<size_t> first(levels.size());
+	std::vector<uint32_t> ntiles(levels.size());
+	std::vector<MipTask> all;
+	for (size_t l = 0; l < levels.size(); ++l) {
+		uint64_t tiles = 0;
+		for (MipTask &t : levels[l]) {
+			t.tile0 = (uint32_t)tiles;
+			const uint64_t ow = t.w > 1 ? t.w >> 1 : 1, oh = t.h > 1 ? t.h >> 1 : 1;
+			tiles += (ow * oh + 255) / 256;
+		}
+		if (tiles > 0x7fffffffull) return set_error(S2TC_E_RANGE, "mip_chain_batch: level %zu too large", l);
+		first[l] = all.size();
+		ntiles[l] = (uint32_t)tiles;
+		all.insert(all.end(), levels[l].begin(), levels[l].end());
+	}
+	if (int r = ensure_device()) return r;
+	hipStream_t st = (hipStream_t)stream;
+	MipTask *d = nullptr;
+	int rc = hip_check(hipMalloc((void **)&d, all.size() * sizeof(MipTask)), "hipMalloc mip tasks");
+	if (!rc)
+		rc = hip_check(hipMemcpyAsync(d, all.data(), all.size() * sizeof(MipTask), hipMemcpyHostToDevice, st),
+		               "H2D mip tasks");
+	for (size_t l = 0; !rc && l < levels.size(); ++l) {
+		k_mip_reduce_batch<<<ntiles[l], 256, 0, st>>>(d + first[l], (int)levels[l].size());
+		rc = hip_check(hipGetLastError(), "launch k_mip_reduce_batch");
+	}
+	// the task list is read by the kernels: keep it until they are done
+	const int rs = hip_check(hipStreamSynchronize(st), "mip_chain_batch synchronize");
+	if (d) hipFree(d);
+	return rc ? rc : rs;
 }

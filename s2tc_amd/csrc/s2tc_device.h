@@ -42,6 +42,22 @@ S2TC_D int texel_a(uint32_t t) { return (int)(t >> 24); }
 S2TC_D int shrr(int a, int n) { return (int)((uint32_t)a + (1u << (n - 1))) >> n; }
 S2TC_D int wmul(int a, int b) { return (int)((uint32_t)a * (uint32_t)b); }
 S2TC_D int wadd(int a, int b) { return (int)((uint32_t)a + (uint32_t)b); }
+// Full-rate 24-bit multiplies for factors known to lie in (-2^23, 2^23):
+// the low 32 bits of a * b (+ c), i.e. the int32 wrap of the reference's
+// expressions.  Written out because the compiler otherwise picks the
+// quarter-rate 32-bit multiplies (v_mul_lo_u32, v_mad_u64_u32) for them.
+S2TC_D int mul_i24(int a, int b)
+{
+	int r;
+	asm("v_mul_i32_i24 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+	return r;
+}
+S2TC_D int mad_i24(int a, int b, int c)
+{
+	int r;
+	asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+	return r;
+}
 
 struct FeatI { int x, y, z; };
 struct FeatF { float x, y, z; };
@@ -84,46 +100,58 @@ template <> struct Dist<CD_WAVG> {
 		return ((dr * dr) << 2) + ((dg * dg) << 2) + (db * db);
 	}
 };
-// YUV / RGB: weighted Y'UV difference (:243-267); no overflow is possible
-// (|y| <= 6259*31, so every term stays far below 2^31).
+// YUV / RGB: weighted Y'UV difference (:243-267).  y, u and v are linear in
+// the component differences, so the feature of a colour is its own
+// (Y, U, V) = (KR r + KG g + KB b, 202 r - Y, 202 b - Y) and the difference
+// of two features is exactly the reference's (y, u, v).  No overflow is
+// possible: |y| <= 6272 and |u|, |v| <= 202 * 31 + 6272 = 12534, so every
+// term stays far below 2^31 (and every factor below 2^23).
 template <int KR, int KG, int KB> struct DistYuvLike {
 	typedef FeatI F;
-	static const bool kCheap = true;
-	static S2TC_D F feat(uint32_t v) { return F{c_r(v), c_g(v), c_b(v)}; }
-	static S2TC_D F featq(uint32_t t) { return F{(int)(t & 31u), (int)((t >> 8) & 63u), (int)((t >> 16) & 31u)}; }
+	static const bool kCheap = true;  // a few integer ops from the texel word
+	static S2TC_D F lin(int r, int g, int b)
+	{
+		const int y = r * KR + g * KG + b * KB;
+		return F{y, r * 202 - y, b * 202 - y};
+	}
+	static S2TC_D F feat(uint32_t v) { return lin(c_r(v), c_g(v), c_b(v)); }
+	static S2TC_D F featq(uint32_t t) { return lin((int)(t & 31u), (int)((t >> 8) & 63u), (int)((t >> 16) & 31u)); }
 	static S2TC_D int d(const F &a, const F &b)
 	{
-		int dr = a.x - b.x, dg = a.y - b.y, db = a.z - b.z;
-		int y = dr * KR + dg * KG + db * KB;
-		int u = dr * 202 - y;
-		int v = db * 202 - y;
-		return ((y * y) << 1) + shrr(u * u, 3) + shrr(v * v, 4);
+		const int y = a.x - b.x, u = a.y - b.y, v = a.z - b.z;
+		// (y*y << 1) + SHRR(u*u, 3) + SHRR(v*v, 4)
+		return (mul_i24(y, y) << 1) + (mad_i24(u, u, 4) >> 3) + (mad_i24(v, v, 8) >> 4);
 	}
 };
 template <> struct Dist<CD_YUV> : DistYuvLike<60, 59, 22> {};
 template <> struct Dist<CD_RGB> : DistYuvLike<42, 72, 14> {};
-// SRGB (:269-283): differences of squared components; SHRR(y,3)*SHRR(y,4)
-// overflows int32 for large differences and the reference wraps, so the
-// products go through uint32.  Not symmetric in its arguments.
+// SRGB (:269-283): y, u and v are linear in the differences of squared
+// components, so the feature of a colour is (Y, U, V) = (84 R + 72 G + 28 B,
+// 409 R - Y, 409 B - Y) with R = r^2, G = g^2, B = b^2 (|Y| <= 393400, no
+// overflow), and a difference of features is the reference's (y, u, v).
+// SHRR(y,3)*SHRR(y,4) overflows int32 for large differences and the
+// reference wraps, so the products wrap too.  Not symmetric in its
+// arguments.  Every SHRR factor is below (409*961 + 393400) / 8 < 2^23.
 template <> struct Dist<CD_SRGB> {
 	typedef FeatI F;
 	static const bool kCheap = false;  // features are kept per texel
 	static S2TC_D F featq(uint32_t t) { return feat(((t & 0x1fu) << 11) | (((t >> 8) & 0x3fu) << 5) | ((t >> 16) & 0x1fu)); }
 	static S2TC_D F feat(uint32_t v)
 	{
-		int r = c_r(v), g = c_g(v), b = c_b(v);
-		return F{r * r, g * g, b * b};
+		const int r = c_r(v), g = c_g(v), b = c_b(v);
+		const int R = r * r, G = g * g, B = b * b;
+		const int y = R * 84 + G * 72 + B * 28;
+		return F{y, R * 409 - y, B * 409 - y};
 	}
 	static S2TC_D int d(const F &a, const F &b)
 	{
-		int dr = a.x - b.x, dg = a.y - b.y, db = a.z - b.z;
-		int y = dr * 84 + dg * 72 + db * 28;
-		int u = dr * 409 - y;
-		int v = db * 409 - y;
-		int sy = wmul(shrr(y, 3), shrr(y, 4));
-		int su = wmul(shrr(u, 3), shrr(u, 4));
-		int sv = wmul(shrr(v, 3), shrr(v, 4));
-		return wadd(wadd(shrr(sy, 4), shrr(su, 8)), shrr(sv, 9));
+		const int y = a.x - b.x, u = a.y - b.y, v = a.z - b.z;
+		// SHRR(SHRR(y,3) * SHRR(y,4), 4) etc., the product and its rounding
+		// term wrapping as in int32
+		const int sy = mad_i24(shrr(y, 3), shrr(y, 4), 8) >> 4;
+		const int su = mad_i24(shrr(u, 3), shrr(u, 4), 128) >> 8;
+		const int sv = mad_i24(shrr(v, 3), shrr(v, 4), 256) >> 9;
+		return wadd(wadd(sy, su), sv);
 	}
 };
 // srgb_get_y (:285-296): (int)(sqrtf(37*(84r^2+72g^2+28b^2)) + 0.5f); the
@@ -141,7 +169,8 @@ __device__ uint16_t s2tc_srgb_y_table[65536];
 #define S2TC_SRGB_Y_TABLE 1
 #endif
 
-// SRGB_MIXED (:298-315): feature (Y, 191r - Y, 191b - Y).
+// SRGB_MIXED (:298-315): feature (Y, 191r - Y, 191b - Y); Y <= 3820, so
+// the differences are below 191*31 + 2*3820 < 2^23.
 template <> struct Dist<CD_SRGB_MIXED> {
 	typedef FeatI F;
 	static const bool kCheap = false;  // features are kept per texel
@@ -161,8 +190,9 @@ template <> struct Dist<CD_SRGB_MIXED> {
 	}
 	static S2TC_D int d(const F &a, const F &b)
 	{
-		int y = a.x - b.x, u = a.y - b.y, v = a.z - b.z;
-		return ((y * y) << 3) + shrr(u * u, 1) + shrr(v * v, 2);
+		const int y = a.x - b.x, u = a.y - b.y, v = a.z - b.z;
+		// (y*y << 3) + SHRR(u*u, 1) + SHRR(v*v, 2)
+		return (mul_i24(y, y) << 3) + (mad_i24(u, u, 1) >> 1) + (mad_i24(v, v, 2) >> 2);
 	}
 };
 #if defined(S2TC_CD) && S2TC_CD == 7

@@ -58,3 +58,31 @@ def test_mip_layout_and_errors(gpu):
         src.free()
     finally:
         d.free()
+
+
+def test_mip_chain_batch_matches_oracle(gpu, oracle):
+    """s2tc_amd_mip_chain_batch: many chains of mixed sizes, one launch per
+    level, each level equal to the oracle's."""
+    rng = np.random.default_rng(23)
+    sizes = SIZES + [(1, 1), (2, 1), (512, 8), (8, 512), (300, 299)]
+    bufs = []
+    try:
+        for w, h in sizes:
+            layout, total = gpu.mip_layout(w, h)
+            rgba = rng.integers(0, 256, w * h * 4).astype(np.uint8)
+            d = gpu.DeviceBuffer(total)
+            gpu.check(gpu.lib().s2tc_amd_memcpy_h2d(d.ptr, rgba.ctypes.data, rgba.nbytes))
+            bufs.append((d, w, h, rgba, layout, total))
+        _say(f"case mip_chain_batch {len(bufs)} chains")
+        gpu.mip_chain_batch([(d.ptr, w, h) for d, w, h, _, _, _ in bufs])
+        for d, w, h, rgba, layout, total in bufs:
+            host = d.to_host(total)
+            exp = oracle.mip_chain(rgba, w, h)
+            assert len(exp) == len(layout)
+            for (o, lw, lh), (ew, eh, lv) in zip(layout, exp):
+                assert (lw, lh) == (ew, eh)
+                assert np.array_equal(host[o:o + lw * lh * 4], lv), (w, h, lw, lh)
+    finally:
+        for b in bufs:
+            b[0].free()
+    assert gpu.lib().s2tc_amd_mip_chain_batch(None, 0, None) == 0
