@@ -22,14 +22,6 @@ namespace s2tc {
 // Upper-triangle index of (i, j), i < j < 16.
 __host__ __device__ constexpr int tri(int i, int j) { return i * 16 - i * (i + 1) / 2 + (j - i - 1); }
 
-// Upper bound of one distance value, for modes whose 16-term pair sums stay
-// below 2^23 (0: no such bound is used).  NORMALMAP: 100000 * |b - a|^2 of
-// two unit vectors (plus float slack); quadratic modes: w . (31, 63, 31)^2.
-template <int CD> struct DistBound { static const uint32_t value = 0; };
-template <> struct DistBound<CD_NORMALMAP> { static const uint32_t value = 400100; };
-template <> struct DistBound<CD_AVG> { static const uint32_t value = 11657; };
-template <> struct DistBound<CD_W0AVG> { static const uint32_t value = 5891; };
-template <> struct DistBound<CD_WAVG> { static const uint32_t value = 20681; };
 
 // D(i, k) = dists[i][k] of the reference, stored as the upper triangle of the
 // symmetric square (diagonal 0).

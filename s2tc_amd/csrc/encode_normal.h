@@ -152,6 +152,88 @@ S2TC_D void pair_scan(const int *rows, int t, int m, int pbw, int qthr, ScanKey 
 	}
 }
 
+// Merge a (sum, pair order) candidate into the running lexicographic minimum.
+S2TC_D void merge_key(uint32_t s, uint32_t order, bool ok, ScanKey &st)
+{
+	const bool better = ok && (s < st.hi || (s == st.hi && order < st.lo));
+	st.hi = better ? s : st.hi;
+	st.lo = better ? order : st.lo;
+}
+
+// The pair scan for rows of non-negative values whose 16-term sums stay
+// below 2^23 (bounded distances, DXT5 alpha).  Within one row-block pair a
+// lane's candidates fold into one 32-bit key per step: (sum << 4 | b) for
+// the partner b of block q > p (the pair order is i*1024 + 16q + b, so the
+// smallest key is the smallest (sum, order)), (sum << 8 | lo << 4 | hi) for
+// pairs inside block p.  The partner's index travels through the DPP row
+// with its row values, all-ones when its row is outside the candidates, so
+// an invalid pair's key is all-ones.  The running (sum, order) minimum is
+// merged once per row-block pair.
+template <int R, int END> struct KeyStepsOff {
+	static S2TC_D void run(const int (&X)[16], const int (&Y)[16], uint32_t w, uint32_t &kmin)
+	{
+		int s = 0;
+#pragma unroll
+		for (int k = 0; k < 16; ++k) {
+			const int y = R ? dpp_i<0x120 + R>(Y[k]) : Y[k];
+			s += X[k] < y ? X[k] : y;
+		}
+		const uint32_t b = R ? dpp_u<0x120 + R>(w) : w;
+		kmin = min(kmin, ((uint32_t)s << 4) | b);
+		KeyStepsOff<R + 1, END>::run(X, Y, w, kmin);
+	}
+};
+template <int END> struct KeyStepsOff<END, END> {
+	static S2TC_D void run(const int (&)[16], const int (&)[16], uint32_t, uint32_t &) {}
+};
+template <int R, int END> struct KeyStepsDiag {
+	static S2TC_D void run(const int (&X)[16], int t, uint32_t w, uint32_t &kmin)
+	{
+		int s = 0;
+#pragma unroll
+		for (int k = 0; k < 16; ++k) {
+			const int y = dpp_i<0x120 + R>(X[k]);
+			s += X[k] < y ? X[k] : y;
+		}
+		const uint32_t b = dpp_u<0x120 + R>(w);
+		const uint32_t tb = b & 15u, lo = min((uint32_t)t, tb), hi = max((uint32_t)t, tb);
+		uint32_t key = ((uint32_t)s << 8) | (lo << 4) | hi;
+		// a partner outside the candidates; R = 8 meets each pair twice, lanes t < 8 keep it
+		key = (b > 15u || (R == 8 && t >= 8)) ? 0xffffffffu : key;
+		kmin = min(kmin, key);
+		KeyStepsDiag<R + 1, END>::run(X, t, w, kmin);
+	}
+};
+template <int END> struct KeyStepsDiag<END, END> {
+	static S2TC_D void run(const int (&)[16], int, uint32_t, uint32_t &) {}
+};
+
+S2TC_D uint32_t reduce_rows_key(const int *rows, int t, int m, int pbw)
+{
+	ScanKey st{0xffffffffu, 0xffffffffu, -1};
+	for (int p = 0; p < pbw; ++p) {
+		int X[16];
+		load_row(rows, 16 * p + t, X);
+		const int i = 16 * p + t;
+		const bool irow = i < m;
+		{
+			uint32_t kmin = 0xffffffffu;
+			KeyStepsDiag<1, 9>::run(X, t, irow ? (uint32_t)t : 0xffffffffu, kmin);
+			const uint32_t lo = 16u * p + ((kmin >> 4) & 15u), hi = 16u * p + (kmin & 15u);
+			merge_key(kmin >> 8, lo * 1024u + hi, irow && kmin != 0xffffffffu, st);
+		}
+		for (int q = p + 1; q < pbw; ++q) {
+			int Y[16];
+			load_row(rows, 16 * q + t, Y);
+			uint32_t kmin = 0xffffffffu;
+			KeyStepsOff<0, 16>::run(X, Y, 16 * q + t < m ? (uint32_t)t : 0xffffffffu, kmin);
+			merge_key(kmin >> 4, (uint32_t)i * 1024u + 16u * q + (kmin & 15u), irow && kmin != 0xffffffffu, st);
+		}
+	}
+	row_min_key(st.hi, st.lo);
+	return st.lo;
+}
+
 // reduce_colors_inplace / _2fixpoints argmin over the rows: returns the
 // winning pair order i*1024+j (group-uniform).
 template <bool NEG>
@@ -411,7 +493,9 @@ S2TC_D void encode_block_normal(uint32_t tex, int t, int w, int h, int dxt, int 
 		for (int k = 0; k < 4; ++k) rp[k] = make_int4(dv[4 * k], dv[4 * k + 1], dv[4 * k + 2], dv[4 * k + 3]);
 	}
 	__syncthreads();
-	const uint32_t cbest = reduce_rows<!IsSymmetric<CD>::value>(sm.rows, t, m, pbw);
+	uint32_t cbest;
+	if constexpr (DistBound<CD>::value != 0) cbest = reduce_rows_key(sm.rows, t, m, pbw);
+	else cbest = reduce_rows<!IsSymmetric<CD>::value>(sm.rows, t, m, pbw);
 	const int ci = min((int)(cbest >> 10), m - 1), cj = min((int)(cbest & 1023u), m - 1);
 	uint32_t c0 = sm.cand[ci] & 0xffffu, c1 = sm.cand[cj] & 0xffffu;
 	int a0 = 0, a1 = 0;
@@ -440,7 +524,7 @@ S2TC_D void encode_block_normal(uint32_t tex, int t, int w, int h, int dxt, int 
 	}
 	__syncthreads();
 	if (dxt == DXT5) {
-		const uint32_t abest = reduce_rows<false>(sm.rows, t, m, pbw);
+		const uint32_t abest = reduce_rows_key(sm.rows, t, m, pbw);
 		const int ai = min((int)(abest >> 10), m - 1), aj = min((int)(abest & 1023u), m - 1);
 		a0 = (int)(sm.cand[ai] >> 16);
 		a1 = (int)(sm.cand[aj] >> 16);

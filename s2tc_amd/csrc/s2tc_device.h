@@ -239,6 +239,15 @@ template <> struct Dist<CD_NORMALMAP> {
 	}
 };
 
+// Upper bound of one distance value, for modes whose 16-term pair sums stay
+// below 2^23 (0: no such bound is used).  NORMALMAP: 100000 * |b - a|^2 of
+// two unit vectors (plus float slack); quadratic modes: w . (31, 63, 31)^2.
+template <int CD> struct DistBound { static const uint32_t value = 0; };
+template <> struct DistBound<CD_NORMALMAP> { static const uint32_t value = 400100; };
+template <> struct DistBound<CD_AVG> { static const uint32_t value = 11657; };
+template <> struct DistBound<CD_W0AVG> { static const uint32_t value = 5891; };
+template <> struct DistBound<CD_WAVG> { static const uint32_t value = 20681; };
+
 template <int CD> struct IsSymmetric { static const bool value = true; };
 
 // Features of the 16 texels of a block: kept in registers for distances
