@@ -160,17 +160,23 @@ S2TC_D void merge_key(uint32_t s, uint32_t order, bool ok, ScanKey &st)
 	st.lo = better ? order : st.lo;
 }
 
-// The pair scan for rows of non-negative values whose 16-term sums stay
-// below 2^23 (bounded distances, DXT5 alpha).  Within one row-block pair a
-// lane's candidates fold into one 32-bit key per step: (sum << 4 | b) for
-// the partner b of block q > p (the pair order is i*1024 + 16q + b, so the
+// The pair scan for rows of non-negative values whose 16-term sums do not
+// wrap (every distance but SRGB, and DXT5 alpha).  Within one row-block pair
+// a lane folds its pairs into one key per step: (sum << 4 | b) for the
+// partner b of block q > p (the pair order is i*1024 + 16q + b, so the
 // smallest key is the smallest (sum, order)), (sum << 8 | lo << 4 | hi) for
 // pairs inside block p.  The partner's index travels through the DPP row
-// with its row values, all-ones when its row is outside the candidates, so
-// an invalid pair's key is all-ones.  The running (sum, order) minimum is
+// with its row values, all-ones when its row is outside the candidates, which
+// makes the key of an invalid pair larger than every valid one.  Keys are 32
+// bits when the sums stay below 2^23 (WIDE false: bounded distances, alpha),
+// 64 bits otherwise (sums below 2^31).  The running (sum, order) minimum is
 // merged once per row-block pair.
-template <int R, int END> struct KeyStepsOff {
-	static S2TC_D void run(const int (&X)[16], const int (&Y)[16], uint32_t w, uint32_t &kmin)
+template <bool WIDE> struct ScanKeyT { typedef uint32_t T; };
+template <> struct ScanKeyT<true> { typedef uint64_t T; };
+
+template <int R, int END, bool WIDE> struct KeyStepsOff {
+	typedef typename ScanKeyT<WIDE>::T K;
+	static S2TC_D void run(const int (&X)[16], const int (&Y)[16], uint32_t w, K &kmin)
 	{
 		int s = 0;
 #pragma unroll
@@ -179,15 +185,22 @@ template <int R, int END> struct KeyStepsOff {
 			s += X[k] < y ? X[k] : y;
 		}
 		const uint32_t b = R ? dpp_u<0x120 + R>(w) : w;
-		kmin = min(kmin, ((uint32_t)s << 4) | b);
-		KeyStepsOff<R + 1, END>::run(X, Y, w, kmin);
+		if constexpr (WIDE) {
+			// an invalid partner (all-ones) sets the high word to >= 2^28 - 1 > s >> 28
+			const uint64_t key = (((uint64_t)(uint32_t)s << 4) | b) | ((uint64_t)(b >> 4) << 32);
+			kmin = min(kmin, key);
+		} else {
+			kmin = min(kmin, ((uint32_t)s << 4) | b);
+		}
+		KeyStepsOff<R + 1, END, WIDE>::run(X, Y, w, kmin);
 	}
 };
-template <int END> struct KeyStepsOff<END, END> {
-	static S2TC_D void run(const int (&)[16], const int (&)[16], uint32_t, uint32_t &) {}
+template <int END, bool WIDE> struct KeyStepsOff<END, END, WIDE> {
+	static S2TC_D void run(const int (&)[16], const int (&)[16], uint32_t, typename ScanKeyT<WIDE>::T &) {}
 };
-template <int R, int END> struct KeyStepsDiag {
-	static S2TC_D void run(const int (&X)[16], int t, uint32_t w, uint32_t &kmin)
+template <int R, int END, bool WIDE> struct KeyStepsDiag {
+	typedef typename ScanKeyT<WIDE>::T K;
+	static S2TC_D void run(const int (&X)[16], int t, uint32_t w, K &kmin)
 	{
 		int s = 0;
 #pragma unroll
@@ -197,19 +210,21 @@ template <int R, int END> struct KeyStepsDiag {
 		}
 		const uint32_t b = dpp_u<0x120 + R>(w);
 		const uint32_t tb = b & 15u, lo = min((uint32_t)t, tb), hi = max((uint32_t)t, tb);
-		uint32_t key = ((uint32_t)s << 8) | (lo << 4) | hi;
+		K key = ((K)(uint32_t)s << 8) | (K)((lo << 4) | hi);
 		// a partner outside the candidates; R = 8 meets each pair twice, lanes t < 8 keep it
-		key = (b > 15u || (R == 8 && t >= 8)) ? 0xffffffffu : key;
+		key = (b > 15u || (R == 8 && t >= 8)) ? ~(K)0 : key;
 		kmin = min(kmin, key);
-		KeyStepsDiag<R + 1, END>::run(X, t, w, kmin);
+		KeyStepsDiag<R + 1, END, WIDE>::run(X, t, w, kmin);
 	}
 };
-template <int END> struct KeyStepsDiag<END, END> {
-	static S2TC_D void run(const int (&)[16], int, uint32_t, uint32_t &) {}
+template <int END, bool WIDE> struct KeyStepsDiag<END, END, WIDE> {
+	static S2TC_D void run(const int (&)[16], int, uint32_t, typename ScanKeyT<WIDE>::T &) {}
 };
 
+template <bool WIDE>
 S2TC_D uint32_t reduce_rows_key(const int *rows, int t, int m, int pbw)
 {
+	typedef typename ScanKeyT<WIDE>::T K;
 	ScanKey st{0xffffffffu, 0xffffffffu, -1};
 	for (int p = 0; p < pbw; ++p) {
 		int X[16];
@@ -217,17 +232,18 @@ S2TC_D uint32_t reduce_rows_key(const int *rows, int t, int m, int pbw)
 		const int i = 16 * p + t;
 		const bool irow = i < m;
 		{
-			uint32_t kmin = 0xffffffffu;
-			KeyStepsDiag<1, 9>::run(X, t, irow ? (uint32_t)t : 0xffffffffu, kmin);
-			const uint32_t lo = 16u * p + ((kmin >> 4) & 15u), hi = 16u * p + (kmin & 15u);
-			merge_key(kmin >> 8, lo * 1024u + hi, irow && kmin != 0xffffffffu, st);
+			K kmin = ~(K)0;
+			KeyStepsDiag<1, 9, WIDE>::run(X, t, irow ? (uint32_t)t : 0xffffffffu, kmin);
+			const uint32_t lo = 16u * p + (uint32_t)((kmin >> 4) & 15u), hi = 16u * p + (uint32_t)(kmin & 15u);
+			merge_key((uint32_t)(kmin >> 8), lo * 1024u + hi, irow && kmin != ~(K)0, st);
 		}
 		for (int q = p + 1; q < pbw; ++q) {
 			int Y[16];
 			load_row(rows, 16 * q + t, Y);
-			uint32_t kmin = 0xffffffffu;
-			KeyStepsOff<0, 16>::run(X, Y, 16 * q + t < m ? (uint32_t)t : 0xffffffffu, kmin);
-			merge_key(kmin >> 4, (uint32_t)i * 1024u + 16u * q + (kmin & 15u), irow && kmin != 0xffffffffu, st);
+			K kmin = ~(K)0;
+			KeyStepsOff<0, 16, WIDE>::run(X, Y, 16 * q + t < m ? (uint32_t)t : 0xffffffffu, kmin);
+			const bool ok = WIDE ? (kmin >> 32) < 8u : kmin != ~(K)0;
+			merge_key((uint32_t)(kmin >> 4), (uint32_t)i * 1024u + 16u * q + (uint32_t)(kmin & 15u), irow && ok, st);
 		}
 	}
 	row_min_key(st.hi, st.lo);
@@ -494,8 +510,9 @@ S2TC_D void encode_block_normal(uint32_t tex, int t, int w, int h, int dxt, int 
 	}
 	__syncthreads();
 	uint32_t cbest;
-	if constexpr (DistBound<CD>::value != 0) cbest = reduce_rows_key(sm.rows, t, m, pbw);
-	else cbest = reduce_rows<!IsSymmetric<CD>::value>(sm.rows, t, m, pbw);
+	if constexpr (DistBound<CD>::value != 0) cbest = reduce_rows_key<false>(sm.rows, t, m, pbw);
+	else if constexpr (IsSymmetric<CD>::value) cbest = reduce_rows_key<true>(sm.rows, t, m, pbw);
+	else cbest = reduce_rows<true>(sm.rows, t, m, pbw);
 	const int ci = min((int)(cbest >> 10), m - 1), cj = min((int)(cbest & 1023u), m - 1);
 	uint32_t c0 = sm.cand[ci] & 0xffffu, c1 = sm.cand[cj] & 0xffffu;
 	int a0 = 0, a1 = 0;
@@ -524,7 +541,7 @@ S2TC_D void encode_block_normal(uint32_t tex, int t, int w, int h, int dxt, int 
 	}
 	__syncthreads();
 	if (dxt == DXT5) {
-		const uint32_t abest = reduce_rows_key(sm.rows, t, m, pbw);
+		const uint32_t abest = reduce_rows_key<false>(sm.rows, t, m, pbw);
 		const int ai = min((int)(abest >> 10), m - 1), aj = min((int)(abest & 1023u), m - 1);
 		a0 = (int)(sm.cand[ai] >> 16);
 		a1 = (int)(sm.cand[aj] >> 16);

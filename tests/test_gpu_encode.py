@@ -133,3 +133,19 @@ def test_uniform_alpha_blocks(gpu, oracle, raw):
                 got = gpu.encode_image(q, w, h, 2, cd, nrandom, refine)
             exp = oracle.encode_quantized(q, w, h, 2, cd, nrandom, refine, 0)
             assert np.array_equal(got, exp), (raw, refine, cd, nrandom)
+
+
+@pytest.mark.parametrize("cd", [4, 5, 6, 7, 2])
+def test_random_candidate_counts(gpu, oracle, cd):
+    """Random-candidate blocks with candidate counts 16 + nrandom that are not
+    multiples of 16 (odd nrandom; DXT1 blocks whose transparent texels leave
+    fewer than 16 colours), up to the GPU limit of 144, every format: the
+    bounded distances (AVG, WAVG, W0AVG, NORMALMAP) and DXT5 alpha take the
+    keyed pair scan, SRGB the per-step scan with its negative-sum rule."""
+    rng = np.random.default_rng(300 + cd)
+    for dxt in range(3):
+        for nr in (1, 15, 17, 31, 48, 144):
+            w, h = (9, 6) if nr < 100 else (5, 4)
+            for kind in ("speckle", "random", "extreme"):
+                q = imgutil.random_quantized(rng, w, h, dxt, kind)
+                _compare(gpu, oracle, q, w, h, dxt, cd, nr, 2 if nr % 2 else 1, 11 + nr, f"candidates-{kind}")

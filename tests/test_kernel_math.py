@@ -144,3 +144,33 @@ def test_uniform_alpha_closed_form(oracle):
             for a in range(256):
                 got = int.from_bytes(out[a, :8].tobytes(), "little")
                 assert got == _uniform_alpha_half(a), (refine, cd, nrandom, a)
+
+
+def test_pair_sums_of_unbounded_distances_never_wrap(oracle):
+    """RGB, YUV and SRGB_MIXED are sums of non-negative terms, and their
+    16-term pair sums stay below 2^31, so the reference's int sums never wrap
+    and its fold "bestsum < 0 || sum < bestsum" is a plain first minimum for
+    them (the kernels fold them as non-negative; only SRGB wraps).  Each
+    distance is a convex quadratic form g of the feature difference, less at
+    most 1 for its rounding shifts, and a convex form takes its largest value
+    over the colours between two vertices of the convex hull of their
+    (Y, U, V) features: so the largest distance is at most the largest
+    vertex-pair distance (from the oracle) + 2."""
+    from scipy.spatial import ConvexHull
+    r, g, b = (a.ravel() for a in np.meshgrid(np.arange(32), np.arange(64), np.arange(32), indexing="ij"))
+    y_srgb = np.floor(np.sqrt((37 * (r * r * 84 + g * g * 72 + b * b * 28)).astype(np.float32))
+                      + np.float32(0.5)).astype(np.int64)
+    feats = {
+        0: (r * 42 + g * 72 + b * 14, 202),   # RGB
+        1: (r * 60 + g * 59 + b * 22, 202),   # YUV
+        3: (y_srgb, 191),                     # SRGB_MIXED
+    }
+    for cd, (y, k) in feats.items():
+        pts = np.stack([y, r * k - y, b * k - y], 1).astype(float)
+        hull = np.unique(ConvexHull(pts).vertices)
+        best = 0
+        for i in hull:
+            for j in hull:
+                best = max(best, oracle.color_dist(cd, (int(r[i]), int(g[i]), int(b[i])),
+                                                   (int(r[j]), int(g[j]), int(b[j]))))
+        assert 16 * (best + 2) < 2 ** 31, (cd, best)
