@@ -204,6 +204,73 @@ def quick_leg(s, size=16384, reps=10):
     return out
 
 
+def calls_leg(s, with_cpu=True, which=("sinus256", "tiles4096", "normal8192")):
+    """The drop-in entry point tx_compress_dxtn on the single-image workloads,
+    timed end to end per call (host buffers in and out: upload, SIMPLE dither,
+    encode, download; synchronous), median of several calls.  The 256x256
+    case is the reference CLI's per-level call, launch-latency-bound; beside
+    it the compiled reference's own tx_compress_dxtn (one host thread, the
+    reference being single-threaded per call) on the same image."""
+    import numpy as np
+    from s2tc_amd import workloads
+    names = ["RGB", "YUV", "SRGB", "SRGB_MIXED", "AVG", "WAVG", "W0AVG", "NORMALMAP"]
+    L = s.lib()
+    L.tx_compress_dxtn.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint,
+                                   ctypes.c_void_p, ctypes.c_int]
+    ref = None
+    if with_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        from oracle import Reference, reference_available
+        ref = Reference() if reference_available() else None
+    saved = {k: os.environ.get(k) for k in ("S2TC_DITHER_MODE", "S2TC_COLORDIST_MODE", "S2TC_RANDOM_COLORS",
+                                            "S2TC_REFINE_COLORS", "S2TC_RANDOM_SEED")}
+    out = []
+    try:
+        for name, reps in (("sinus256", 50), ("tiles4096", 5), ("normal8192", 3)):
+            if name not in which:
+                continue
+            wl = workloads.IMAGE_WORKLOADS[name]
+            src = workloads.make_image_source(wl)
+            w, h = wl["width"], wl["height"]
+            os.environ.update(S2TC_DITHER_MODE=["NONE", "SIMPLE", "FLOYDSTEINBERG"][wl["dither"]],
+                              S2TC_COLORDIST_MODE=names[wl["cd"]], S2TC_RANDOM_COLORS=str(wl["nrandom"]),
+                              S2TC_REFINE_COLORS=["NEVER", "ALWAYS", "LOOP"][wl["refine"]],
+                              S2TC_RANDOM_SEED=str(wl["seed"]))
+            bs = 8 if wl["dxt"] == 0 else 16
+            bw, bh = (w + 3) // 4, (h + 3) // 4
+            dst = np.zeros(bw * bh * bs, np.uint8)
+            fmt = {0: 0x83F1, 1: 0x83F2, 2: 0x83F3}[wl["dxt"]]
+
+            def call(lib):
+                lib.tx_compress_dxtn(wl["srccomps"], w, h, src.ctypes.data, fmt, dst.ctypes.data, bw * bs)
+            call(L)
+            ts = []
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                call(L)
+                ts.append(time.perf_counter() - t0)
+            t = float(np.median(ts))
+            row = dict(workload=name, entry="tx_compress_dxtn", texels=w * h, us_per_call=round(t * 1e6, 1),
+                       mtexels_s_end_to_end=round(w * h / t / 1e6, 1), calls=reps)
+            if ref is not None and name == "sinus256":
+                call(ref.lib)
+                rs = []
+                for _ in range(10):
+                    t0 = time.perf_counter()
+                    call(ref.lib)
+                    rs.append(time.perf_counter() - t0)
+                row["cpu_reference_us_per_call"] = round(float(np.median(rs)) * 1e6, 1)
+            log(f"calls {row}")
+            out.append(row)
+    finally:
+        for k, v in saved.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+    return out
+
+
 def side_leg(s, reps=10):
     """The kernels beside the encoder, each HBM-bound, timed with HIP events:
     one mip halving of a 16384x16384 RGBA8 image (k_mip_reduce: 1 GiB read,
@@ -335,6 +402,10 @@ def main():
     ap.add_argument("--no-quick", action="store_true")
     ap.add_argument("--quick-size", type=int, default=16384)
     ap.add_argument("--no-side", action="store_true")
+    ap.add_argument("--no-calls", action="store_true")
+    ap.add_argument("--only-calls", action="store_true",
+                    help="only time tx_compress_dxtn per call on the single-image workloads and print their JSON")
+    ap.add_argument("--calls-workloads", default="sinus256,tiles4096,normal8192")
     ap.add_argument("--only-side", action="store_true",
                     help="only run the mip / transcode kernels and print their JSON")
     ap.add_argument("--only-quick", action="store_true",
@@ -360,6 +431,10 @@ def main():
             s.check(L.s2tc_amd_memcpy_d2d_kernel(dst.ptr, src.ptr, n, None))
         s.synchronize()
         print(json.dumps({"calibrate_copy_bytes": n}))
+        return
+
+    if a.only_calls:
+        print(json.dumps({"calls": calls_leg(s, not a.no_cpu, a.calls_workloads.split(","))}), flush=True)
         return
 
     if a.only_side:
@@ -466,6 +541,10 @@ def main():
     if not a.no_side and a.workload == "batch":
         side = side_leg(s)
 
+    calls = None
+    if not a.no_calls and a.workload == "batch":
+        calls = calls_leg(s, not a.no_cpu)
+
     serve = None
     if not a.no_serve and a.workload == "batch":
         serve = serve_leg(s, (2000, 10000, 50000), a.serve_duration)
@@ -494,6 +573,7 @@ def main():
         "classes": class_summary,
         "quick_mode": quick,
         "side_kernels": side,
+        "calls": calls,
         "cpu_baseline": cpu,
         "serve": serve,
         "quality": quality,

@@ -7,15 +7,18 @@
 // diffuse() with shift s keeps d in [-(2^s - 1), 2^s - 1] (15 states for
 // shift 3, 7 for shift 2, 31 for shift 4), and diffuse1() keeps
 // e = d + 127 = (127 + sum of inputs) mod 255.  The chain is therefore a
-// scan over transition maps:
+// scan over transition maps, all channels of a call in each launch
+// (blockIdx.y = channel):
 //   1. chunk maps:   for every chunk of L texels and every start state, the
 //                    end state (one thread per (chunk, state));
-//   2. super maps:   compositions of K consecutive chunk maps;
+//   2. super maps:   compositions of K consecutive chunk maps (staged in LDS);
 //   3. super starts: one thread walks the super maps from the initial state;
-//   4. chunk starts: one thread per super chunk walks its K chunk maps;
-//   5. emit:         one thread per chunk re-runs its texels from the start
-//                    state and writes the quantized values.
-// The mod-255 alpha channel uses sums instead of maps.
+//   4. emit:         one workgroup per super chunk: thread k walks the chunk
+//                    maps before its chunk (in LDS) to its start state, then
+//                    re-runs its texels and writes the quantized values.
+// L (16..256) and K (16..256) grow with the image so that small images get
+// short serial walks and many threads.  The mod-255 alpha channel uses sums
+// instead of maps.
 //
 // DITHER_FLOYDSTEINBERG (:1350-1412) diffuses a 12-bit error to the right
 // (7/16) and to the row below (3, 5 and the rest of the remainder).  Texel
@@ -36,8 +39,7 @@
 
 namespace s2tc {
 
-#define DITHER_L 256   // texels per chunk
-#define DITHER_K 256   // chunks per super chunk
+#define DITHER_S 32    // map stride: states per chunk map (at most 31 used)
 
 struct DChan {
 	int off;     // byte offset of the channel within a source texel
@@ -56,115 +58,165 @@ __device__ __forceinline__ int diffuse_step(int d, int src, int shift, int &ret)
 	return s - ((r << shift) | (r >> (8 - 2 * shift)));
 }
 
-// 1. end state of every (chunk, start state)
-__global__ void k_dither_chunk_maps(const uint8_t *__restrict__ src, int srccomps, long long n, DChan ch,
-                                    uint8_t *__restrict__ maps, long long nchunks)
+// The SIMPLE channels of one call and the chunking.
+struct DChans {
+	DChan ch[4];
+	int nch;
+	int L, K;             // texels per chunk, chunks per super chunk
+	long long nchunks, nsuper;
+	long long stride;     // scratch bytes per channel
+};
+
+// per-channel scratch: chunk maps (nchunks x 32), super maps (nsuper x 32),
+// super starts (nsuper)
+__device__ __forceinline__ uint8_t *chan_maps(uint8_t *scr, const DChans &cs, int c) { return scr + c * cs.stride; }
+__device__ __forceinline__ uint8_t *chan_smaps(uint8_t *scr, const DChans &cs, int c)
 {
+	return scr + c * cs.stride + cs.nchunks * DITHER_S;
+}
+__device__ __forceinline__ uint8_t *chan_sstart(uint8_t *scr, const DChans &cs, int c)
+{
+	return scr + c * cs.stride + (cs.nchunks + cs.nsuper) * DITHER_S;
+}
+
+// 1. end state of every (chunk, start state); the texel loads of a chunk are
+// independent of the state, eight at a time ahead of the chain
+__global__ __launch_bounds__(256) void k_dither_chunk_maps(const uint8_t *__restrict__ src, int srccomps, long long n,
+                                                           DChans cs, uint8_t *__restrict__ scr)
+{
+	const DChan ch = cs.ch[blockIdx.y];
 	const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	const long long c = tid >> 5;
+	const int s0 = (int)(tid & 31);
 	const int S = ch.nstates;
-	if (tid >= nchunks * S) return;
-	const long long c = tid / S;
-	const int s0 = (int)(tid - c * S);
-	const long long p0 = c * DITHER_L, p1 = min(p0 + DITHER_L, n);
+	if (c >= cs.nchunks || s0 >= S) return;
+	uint8_t *maps = chan_maps(scr, cs, blockIdx.y);
+	const long long p0 = c * cs.L, p1 = min(p0 + cs.L, n);
+	const uint8_t *sp = src + ch.off;
 	if (ch.kind == 1) {
 		int acc = 0;
-		for (long long p = p0; p < p1; ++p) acc += src[p * srccomps + ch.off];
-		maps[c] = (uint8_t)(acc % 255);
+		for (long long p = p0; p < p1; ++p) acc += sp[p * srccomps];
+		maps[c * DITHER_S] = (uint8_t)(acc % 255);
 		return;
 	}
 	const int half = (S - 1) / 2;
 	int d = s0 - half, ret;
-	for (long long p = p0; p < p1; ++p) d = diffuse_step(d, src[p * srccomps + ch.off], ch.shift, ret);
-	maps[c * S + s0] = (uint8_t)(d + half);
-}
-
-// 2. composition of K consecutive chunk maps
-__global__ void k_dither_super_maps(const uint8_t *__restrict__ maps, long long nchunks, DChan ch,
-                                    uint8_t *__restrict__ smaps, long long nsuper)
-{
-	const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-	const int S = ch.nstates;
-	if (tid >= nsuper * S) return;
-	const long long sc = tid / S;
-	const int s0 = (int)(tid - sc * S);
-	const long long c0 = sc * DITHER_K, c1 = min(c0 + DITHER_K, nchunks);
-	if (ch.kind == 1) {
-		int acc = 0;
-		for (long long c = c0; c < c1; ++c) acc += maps[c];
-		smaps[sc] = (uint8_t)(acc % 255);
-		return;
+	long long p = p0;
+	for (; p + 8 <= p1; p += 8) {
+		int v[8];
+#pragma unroll
+		for (int u = 0; u < 8; ++u) v[u] = sp[(p + u) * srccomps];
+#pragma unroll
+		for (int u = 0; u < 8; ++u) d = diffuse_step(d, v[u], ch.shift, ret);
 	}
-	int s = s0;
-	for (long long c = c0; c < c1; ++c) s = maps[c * S + s];
-	smaps[sc * S + s0] = (uint8_t)s;
+	for (; p < p1; ++p) d = diffuse_step(d, sp[p * srccomps], ch.shift, ret);
+	maps[c * DITHER_S + s0] = (uint8_t)(d + half);
 }
 
-// 3. start state of every super chunk (one thread; super maps staged in LDS)
-__global__ void k_dither_super_starts(const uint8_t *__restrict__ smaps, long long nsuper, DChan ch,
-                                      uint8_t *__restrict__ sstart)
+// 2. composition of the K chunk maps of every super chunk; a workgroup of
+// 128 threads stages the maps of 4 super chunks in LDS, thread (g, state)
+// walks super chunk g
+__global__ __launch_bounds__(128) void k_dither_super_maps(DChans cs, uint8_t *__restrict__ scr)
 {
 	extern __shared__ uint8_t lds[];
-	const int S = ch.nstates;
-	const long long total = nsuper * S;
-	for (long long i = threadIdx.x; i < total; i += blockDim.x) lds[i] = smaps[i];
+	const DChan ch = cs.ch[blockIdx.y];
+	const uint8_t *maps = chan_maps(scr, cs, blockIdx.y);
+	uint8_t *smaps = chan_smaps(scr, cs, blockIdx.y);
+	const long long sc0 = (long long)blockIdx.x * 4;
+	const long long c0 = sc0 * cs.K, c1 = min(c0 + 4LL * cs.K, cs.nchunks);
+	const long long bytes = (c1 - c0) * DITHER_S;
+	const uint32_t *g = (const uint32_t *)(maps + c0 * DITHER_S);
+	for (long long i = threadIdx.x; i < bytes / 4; i += blockDim.x) ((uint32_t *)lds)[i] = g[i];
 	__syncthreads();
-	if (threadIdx.x != 0) return;
-	int s = ch.kind == 1 ? 127 : (S - 1) / 2;  // d = 0 (e = 127 for the mod-255 channel)
-	for (long long sc = 0; sc < nsuper; ++sc) {
-		sstart[sc] = (uint8_t)s;
-		s = ch.kind == 1 ? (s + lds[sc]) % 255 : lds[sc * S + s];
-	}
-}
-
-// 3'. as k_dither_super_starts, reading the super maps from global memory
-__global__ void k_dither_super_starts_global(const uint8_t *__restrict__ smaps, long long nsuper, DChan ch,
-                                             uint8_t *__restrict__ sstart)
-{
-	if (threadIdx.x != 0) return;
+	const int gi = threadIdx.x >> 5, s0 = threadIdx.x & 31;
+	const long long sc = sc0 + gi;
+	if (sc >= cs.nsuper) return;
 	const int S = ch.nstates;
-	int s = ch.kind == 1 ? 127 : (S - 1) / 2;
-	for (long long sc = 0; sc < nsuper; ++sc) {
-		sstart[sc] = (uint8_t)s;
-		s = ch.kind == 1 ? (s + smaps[sc]) % 255 : smaps[sc * S + s];
-	}
-}
-
-// 4. start state of every chunk
-__global__ void k_dither_chunk_starts(const uint8_t *__restrict__ maps, long long nchunks, DChan ch,
-                                      const uint8_t *__restrict__ sstart, long long nsuper,
-                                      uint8_t *__restrict__ cstart)
-{
-	const long long sc = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-	if (sc >= nsuper) return;
-	const int S = ch.nstates;
-	int s = sstart[sc];
-	const long long c0 = sc * DITHER_K, c1 = min(c0 + DITHER_K, nchunks);
-	for (long long c = c0; c < c1; ++c) {
-		cstart[c] = (uint8_t)s;
-		s = ch.kind == 1 ? (s + maps[c]) % 255 : maps[c * S + s];
-	}
-}
-
-// 5. re-run every chunk from its start state and write the channel
-__global__ void k_dither_emit(const uint8_t *__restrict__ src, int srccomps, long long n, DChan ch,
-                              const uint8_t *__restrict__ cstart, long long nchunks, uint8_t *__restrict__ out)
-{
-	const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-	if (c >= nchunks) return;
-	const long long p0 = c * DITHER_L, p1 = min(p0 + DITHER_L, n);
+	const long long a = (long long)gi * cs.K, b = min(a + cs.K, c1 - c0);
 	if (ch.kind == 1) {
-		int e = cstart[c];
+		if (s0) return;
+		int acc = 0;
+		for (long long c = a; c < b; ++c) acc += lds[c * DITHER_S];
+		smaps[sc * DITHER_S] = (uint8_t)(acc % 255);
+		return;
+	}
+	if (s0 >= S) return;
+	int st = s0;
+	for (long long c = a; c < b; ++c) st = lds[c * DITHER_S + st];
+	smaps[sc * DITHER_S + s0] = (uint8_t)st;
+}
+
+// 3. start state of every super chunk: one thread per channel walks the
+// super maps (staged in LDS when they fit)
+__global__ __launch_bounds__(256) void k_dither_super_starts(DChans cs, uint8_t *__restrict__ scr, int staged)
+{
+	extern __shared__ uint8_t lds[];
+	const DChan ch = cs.ch[blockIdx.y];
+	const uint8_t *smaps = chan_smaps(scr, cs, blockIdx.y);
+	uint8_t *sstart = chan_sstart(scr, cs, blockIdx.y);
+	const uint8_t *m = smaps;
+	if (staged) {
+		const long long words = cs.nsuper * DITHER_S / 4;
+		for (long long i = threadIdx.x; i < words; i += blockDim.x) ((uint32_t *)lds)[i] = ((const uint32_t *)smaps)[i];
+		__syncthreads();
+		m = lds;
+	}
+	if (threadIdx.x != 0) return;
+	const int S = ch.nstates;
+	int st = ch.kind == 1 ? 127 : (S - 1) / 2;  // d = 0 (e = 127 for the mod-255 channel)
+	for (long long sc = 0; sc < cs.nsuper; ++sc) {
+		sstart[sc] = (uint8_t)st;
+		st = ch.kind == 1 ? (st + m[sc * DITHER_S]) % 255 : m[sc * DITHER_S + st];
+	}
+}
+
+// 4. one workgroup (K threads) per super chunk: thread k finds the start
+// state of chunk k of the super chunk from the staged chunk maps, then
+// re-runs the chunk's texels and writes the channel
+__global__ __launch_bounds__(256) void k_dither_emit(const uint8_t *__restrict__ src, int srccomps, long long n,
+                                                     DChans cs, uint8_t *__restrict__ scr, uint8_t *__restrict__ out)
+{
+	extern __shared__ uint8_t lds[];
+	const DChan ch = cs.ch[blockIdx.y];
+	const uint8_t *maps = chan_maps(scr, cs, blockIdx.y);
+	const uint8_t *sstart = chan_sstart(scr, cs, blockIdx.y);
+	const long long sc = blockIdx.x;
+	const long long c0 = sc * cs.K, c1 = min(c0 + cs.K, cs.nchunks);
+	const long long words = (c1 - c0) * DITHER_S / 4;
+	const uint32_t *g = (const uint32_t *)(maps + c0 * DITHER_S);
+	for (long long i = threadIdx.x; i < words; i += blockDim.x) ((uint32_t *)lds)[i] = g[i];
+	__syncthreads();
+	const long long c = c0 + threadIdx.x;
+	if (c >= c1) return;
+	int st = sstart[sc];
+	for (int k = 0; k < (int)threadIdx.x; ++k)
+		st = ch.kind == 1 ? (st + lds[k * DITHER_S]) % 255 : lds[k * DITHER_S + st];
+	const long long p0 = c * cs.L, p1 = min(p0 + cs.L, n);
+	const uint8_t *sp = src + ch.off;
+	if (ch.kind == 1) {
+		int e = st;
 		for (long long p = p0; p < p1; ++p) {
-			const int v = src[p * srccomps + ch.off];
+			const int v = sp[p * srccomps];
 			out[p * 4 + ch.off] = (uint8_t)(e + v >= 255 ? 1 : 0);  // d + src >= 128 with e = d + 127
 			e = (e + v) % 255;
 		}
 		return;
 	}
 	const int half = (ch.nstates - 1) / 2;
-	int d = cstart[c] - half, ret;
-	for (long long p = p0; p < p1; ++p) {
-		d = diffuse_step(d, src[p * srccomps + ch.off], ch.shift, ret);
+	int d = st - half, ret;
+	long long p = p0;
+	for (; p + 8 <= p1; p += 8) {
+		int v[8];
+#pragma unroll
+		for (int u = 0; u < 8; ++u) v[u] = sp[(p + u) * srccomps];
+#pragma unroll
+		for (int u = 0; u < 8; ++u) {
+			d = diffuse_step(d, v[u], ch.shift, ret);
+			out[(p + u) * 4 + ch.off] = (uint8_t)ret;
+		}
+	}
+	for (; p < p1; ++p) {
+		d = diffuse_step(d, sp[p * srccomps], ch.shift, ret);
 		out[p * 4 + ch.off] = (uint8_t)ret;
 	}
 }
@@ -322,39 +374,47 @@ static int launch_rgb565_fs(const uint8_t *src, int srccomps, int w, int h, int 
 
 // ---- host launchers ----
 
-static const int kMaxStates = 31;
+// Chunk length L and super chunk size K for n texels: powers of two with
+// about n / 16384 texels per chunk (16..256) and about sqrt(chunks) chunks per
+// super chunk (16..256).
+static void simple_chunking(long long n, DChans &cs)
+{
+	int L = 16;
+	while (L < 256 && (long long)L * 16384 < n) L <<= 1;
+	const long long nchunks = (n + L - 1) / L;
+	int K = 16;
+	while (K < 256 && (long long)K * K < nchunks) K <<= 1;
+	cs.L = L;
+	cs.K = K;
+	cs.nchunks = nchunks;
+	cs.nsuper = (nchunks + K - 1) / K;
+	// 4-byte aligned regions (the kernels copy maps as words)
+	cs.stride = ((nchunks + cs.nsuper) * DITHER_S + cs.nsuper + 255) & ~255LL;
+}
 
 size_t rgb565_scratch_bytes(int w, int h)
 {
 	const long long n = (long long)w * h;
-	const long long nchunks = (n + DITHER_L - 1) / DITHER_L;
-	const long long nsuper = (nchunks + DITHER_K - 1) / DITHER_K;
-	const size_t simple = (size_t)(nchunks * (kMaxStates + 1) + nsuper * (kMaxStates + 1) + 256);
+	DChans cs;
+	simple_chunking(n > 0 ? n : 1, cs);
+	const size_t simple = (size_t)(4 * cs.stride + 256);
 	const size_t fs = (size_t)(4 * (w + 2)) * sizeof(int) + 256;
 	return simple > fs ? simple : fs;
 }
 
-static int run_simple_channel(const uint8_t *src, int srccomps, long long n, DChan ch, uint8_t *out,
-                              uint8_t *scratch, hipStream_t st)
+static int run_simple(const uint8_t *src, int srccomps, long long n, DChans cs, uint8_t *out, uint8_t *scratch,
+                      hipStream_t st)
 {
-	const long long nchunks = (n + DITHER_L - 1) / DITHER_L;
-	const long long nsuper = (nchunks + DITHER_K - 1) / DITHER_K;
-	const int S = ch.nstates;
-	uint8_t *maps = scratch;
-	uint8_t *smaps = maps + nchunks * S;
-	uint8_t *sstart = smaps + nsuper * S;
-	uint8_t *cstart = sstart + nsuper;
-	const int T = 256;
-	k_dither_chunk_maps<<<(unsigned)((nchunks * S + T - 1) / T), T, 0, st>>>(src, srccomps, n, ch, maps, nchunks);
-	k_dither_super_maps<<<(unsigned)((nsuper * S + T - 1) / T), T, 0, st>>>(maps, nchunks, ch, smaps, nsuper);
-	const size_t lds = (size_t)(nsuper * S);
-	if (lds <= 60000) {
-		k_dither_super_starts<<<1, T, lds, st>>>(smaps, nsuper, ch, sstart);
-	} else {
-		k_dither_super_starts_global<<<1, 64, 0, st>>>(smaps, nsuper, ch, sstart);
-	}
-	k_dither_chunk_starts<<<(unsigned)((nsuper + T - 1) / T), T, 0, st>>>(maps, nchunks, ch, sstart, nsuper, cstart);
-	k_dither_emit<<<(unsigned)((nchunks + T - 1) / T), T, 0, st>>>(src, srccomps, n, ch, cstart, nchunks, out);
+	simple_chunking(n, cs);
+	const unsigned nch = (unsigned)cs.nch;
+	k_dither_chunk_maps<<<dim3((unsigned)((cs.nchunks * DITHER_S + 255) / 256), nch), 256, 0, st>>>(src, srccomps, n,
+	                                                                                                  cs, scratch);
+	k_dither_super_maps<<<dim3((unsigned)((cs.nsuper + 3) / 4), nch), 128, (size_t)4 * cs.K * DITHER_S, st>>>(cs, scratch);
+	const size_t lds = (size_t)(cs.nsuper * DITHER_S);
+	const int staged = lds <= 60000;
+	k_dither_super_starts<<<dim3(1, nch), 256, staged ? lds : 0, st>>>(cs, scratch, staged);
+	k_dither_emit<<<dim3((unsigned)cs.nsuper, nch), (unsigned)cs.K, (size_t)cs.K * DITHER_S, st>>>(src, srccomps, n, cs,
+	                                                                                             scratch, out);
 	return hip_check(hipGetLastError(), "dither SIMPLE kernels");
 }
 
@@ -380,12 +440,15 @@ int launch_rgb565(const void *d_src, int srccomps, int w, int h, int alphabits, 
 	if (dither == DITHER_FLOYDSTEINBERG)
 		return launch_rgb565_fs(src, srccomps, w, h, alphabits, out, d_scratch, st);
 	uint8_t *scr = (uint8_t *)d_scratch;
-	const DChan cr{0, 0, 3, 15}, cg{1, 0, 2, 7}, cb{2, 0, 3, 15};
-	if (int r = run_simple_channel(src, srccomps, n, cr, out, scr, st)) return r;
-	if (int r = run_simple_channel(src, srccomps, n, cg, out, scr, st)) return r;
-	if (int r = run_simple_channel(src, srccomps, n, cb, out, scr, st)) return r;
-	if (srccomps == 4 && alphabits == 1) return run_simple_channel(src, srccomps, n, DChan{3, 1, 0, 1}, out, scr, st);
-	if (srccomps == 4 && alphabits == 4) return run_simple_channel(src, srccomps, n, DChan{3, 0, 4, 31}, out, scr, st);
+	DChans cs;
+	cs.ch[0] = DChan{0, 0, 3, 15};
+	cs.ch[1] = DChan{1, 0, 2, 7};
+	cs.ch[2] = DChan{2, 0, 3, 15};
+	cs.nch = 3;
+	if (srccomps == 4 && alphabits == 1) cs.ch[cs.nch++] = DChan{3, 1, 0, 1};
+	else if (srccomps == 4 && alphabits == 4) cs.ch[cs.nch++] = DChan{3, 0, 4, 31};
+	if (int r = run_simple(src, srccomps, n, cs, out, scr, st)) return r;
+	if (cs.nch == 4) return 0;
 	k_quantize_none<<<grid, 256, 0, st>>>(src, srccomps, n, alphabits, 0, 1, out);
 	return hip_check(hipGetLastError(), "quantize alpha kernel");
 }
