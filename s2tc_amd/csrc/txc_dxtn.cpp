@@ -37,11 +37,45 @@ struct DevBuf {
 	}
 };
 
+// Pinned host memory (hipHostMalloc): DMA-able staging for the host-buffer
+// entry points, grown on demand and kept with the context.
+struct HostBuf {
+	void *p = nullptr;
+	size_t cap = 0;
+	int ensure(size_t n)
+	{
+		if (n <= cap) return 0;
+		if (p) (void)hipHostFree(p);
+		p = nullptr;
+		cap = 0;
+		n = (n + 4095) & ~(size_t)4095;
+		if (int r = hip_check(hipHostMalloc(&p, n, hipHostMallocDefault), "hipHostMalloc")) return r;
+		cap = n;
+		return 0;
+	}
+	uint8_t *data() const { return (uint8_t *)p; }
+};
+
+// Inputs up to this size are copied into pinned staging before the upload;
+// larger ones go to the runtime's pageable copy path.
+static const size_t kStageMax = 8u << 20;
+
 struct Ctx {
 	hipStream_t stream = nullptr;
 	DevBuf src, q, out, scratch;
-	std::vector<uint8_t> host_out;
+	HostBuf in_stage, host_out;
 };
+
+// host -> device upload of a caller's buffer on the context's stream
+static int upload(Ctx *c, void *dst, const void *src, size_t n)
+{
+	if (n <= kStageMax) {
+		if (int r = c->in_stage.ensure(n)) return r;
+		memcpy(c->in_stage.p, src, n);
+		src = c->in_stage.p;
+	}
+	return hip_check(hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, c->stream), "H2D");
+}
 
 static std::mutex g_pool_mu;
 static std::vector<Ctx *> g_pool;
@@ -125,8 +159,7 @@ static int encode_host_image(Ctx *c, int srccomps, int width, int height, const 
 	const int alphabits = dxt == DXT1 ? 1 : dxt == DXT3 ? 4 : 8;
 	if (int r = c->src.ensure(n * srccomps)) return r;
 	if (int r = c->out.ensure(obytes)) return r;
-	if (int r = hip_check(hipMemcpyAsync(c->src.p, src, n * srccomps, hipMemcpyHostToDevice, c->stream), "H2D"))
-		return r;
+	if (int r = upload(c, c->src.p, src, n * srccomps)) return r;
 	EncodeArgs a{};
 	a.dst = c->out.p;
 	a.width = width;
@@ -149,7 +182,7 @@ static int encode_host_image(Ctx *c, int srccomps, int width, int height, const 
 		a.src_kind = S2TC_SRC_QUANTIZED;
 	}
 	if (int r = launch_encode(a, c->stream)) return r;
-	c->host_out.resize(obytes);
+	if (int r = c->host_out.ensure(obytes)) return r;
 	if (int r = hip_check(hipMemcpyAsync(c->host_out.data(), c->out.p, obytes, hipMemcpyDeviceToHost, c->stream),
 	                      "D2H"))
 		return r;
@@ -291,7 +324,7 @@ void rgb565_image(unsigned char *out, const unsigned char *rgba, int w, int h, i
 	int r = c->src.ensure(n * sc);
 	if (!r) r = c->q.ensure(n * 4);
 	if (!r) r = c->scratch.ensure(rgb565_scratch_bytes(w, h));
-	if (!r) r = hip_check(hipMemcpyAsync(c->src.p, rgba, n * sc, hipMemcpyHostToDevice, c->stream), "H2D");
+	if (!r) r = upload(c, c->src.p, rgba, n * sc);
 	if (!r) r = launch_rgb565(c->src.p, sc, w, h, alphabits, (int)dither, c->q.p, c->scratch.p, c->stream);
 	if (!r) r = hip_check(hipMemcpyAsync(out, c->q.p, n * 4, hipMemcpyDeviceToHost, c->stream), "D2H");
 	if (!r) r = hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
