@@ -511,6 +511,15 @@ def main():
         if dom["dxt"] == 2:
             scans += alpha_scan_blocks(imgs, dom["cd"], dom["refine"])
         ops = 16 * 16 * 15 * scans
+    elif dom["normal"] == 1:
+        # the same floor with m = 16 + nrandom candidates for every job of the
+        # class (its distance, nrandom > 0), doubled for DXT5 alpha
+        ops = 0
+        for it in imgs.items:
+            if it["cd"] == dom["cd"] and it["nrandom"] > 0:
+                m = 16 + it["nrandom"]
+                ops += s.nblocks(it["width"], it["height"]) * 16 * m * (m - 1) * (2 if it["dxt"] == 2 else 1)
+    if dom["normal"] in (1, 2):
         tops = ops / (dom["ms"] * 1e-3) / 1e12
         roofline["valu"] = {"bound": "valu", "achieved": round(tops, 3), "peak": VALU_PEAK_TOPS,
                             "unit": "Tlane-op/s", "frac": round(tops / VALU_PEAK_TOPS, 4),
