@@ -54,6 +54,7 @@ class Oracle(_Base):
         L.oracle_rgb565_image.argtypes = [P, P, I, I, I, I, I]
         L.oracle_compress_image.argtypes = [I, I, I, P, ctypes.c_uint, P, I, I, I, I, I, U64]
         L.oracle_fetch_texel.argtypes = [ctypes.c_uint, I, P, I, I, P]
+        L.oracle_decode_image.argtypes = [ctypes.c_uint, I, I, P, P]
         L.oracle_mip_reduce.argtypes = [P, P, ctypes.POINTER(I), ctypes.POINTER(I)]
         L.oracle_color_dist.argtypes = [I] * 7
         L.oracle_srgb_get_y.argtypes = [I, I, I]
@@ -92,6 +93,13 @@ class Oracle(_Base):
         t = np.zeros(4, np.uint8)
         self.lib.oracle_fetch_texel(fmt, row_stride_texels, _ptr(blocks), i, j, _ptr(t))
         return t
+
+    def decode_image(self, blocks, fmt, width, height):
+        """fetch_2d_texel_* of every texel, row-major RGBA8 (width*height*4 bytes)."""
+        blocks = np.ascontiguousarray(blocks, dtype=np.uint8)
+        out = np.zeros(width * height * 4, np.uint8)
+        self.lib.oracle_decode_image(fmt, width, height, _ptr(blocks), _ptr(out))
+        return out
 
     def color_dist(self, cd, a, b):
         return self.lib.oracle_color_dist(cd, *a, *b)

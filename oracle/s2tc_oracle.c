@@ -788,6 +788,14 @@ void oracle_fetch_texel(unsigned fmt, int srcRowStride, const uint8_t *pix, int 
 	}
 }
 
+/* every texel of a width x height image through oracle_fetch_texel, row-major RGBA8 */
+void oracle_decode_image(unsigned fmt, int width, int height, const uint8_t *pix, uint8_t *out)
+{
+	int i, j;
+	for (j = 0; j < height; ++j)
+		for (i = 0; i < width; ++i) oracle_fetch_texel(fmt, width, pix, i, j, out + ((size_t)j * width + i) * 4);
+}
+
 /* ---- total colour distance of an encoding (test metric) ---- */
 int64_t oracle_colordist_error(const uint8_t *q, int width, int height, int dxt, int cd, const uint8_t *blocks)
 {

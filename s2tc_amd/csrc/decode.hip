@@ -24,6 +24,23 @@ __device__ __forceinline__ uint32_t expand565(uint32_t c)
 	return ((r << 3) | (r >> 2)) | (((g << 2) | (g >> 4)) << 8) | (((b << 3) | (b >> 2)) << 16);
 }
 
+// DXT5 alpha palette slot for (index ab, texel odd, 6-value mode) at bit
+// 2 * (ab * 4 + odd * 2 + six): 0 = a0, 1 = a1, 2 = 0, 3 = 255.
+constexpr uint64_t alpha_slots()
+{
+	uint64_t t = 0;
+	for (unsigned ab = 0; ab < 8; ++ab)
+		for (unsigned odd = 0; odd < 2; ++odd)
+			for (unsigned six = 0; six < 2; ++six) {
+				unsigned slot = ab == 0 ? 0u : ab == 1 ? 1u : odd ? 1u : 0u;
+				if (six && ab == 6) slot = 2;
+				if (six && ab == 7) slot = 3;
+				t |= (uint64_t)slot << (2 * (ab * 4 + odd * 2 + six));
+			}
+	return t;
+}
+constexpr uint64_t kAlphaSlots = alpha_slots();
+
 __global__ __launch_bounds__(256) void k_decode(const uint8_t *__restrict__ blocks, unsigned fmt, int width,
                                                 int height, uint32_t *__restrict__ rgba)
 {
@@ -49,16 +66,18 @@ __global__ __launch_bounds__(256) void k_decode(const uint8_t *__restrict__ bloc
 		const uint32_t q = (blk[(j & 3) * 2 + ((i & 3) >> 1)] >> (4 * (i & 1))) & 15u;
 		a = q | (q << 4);
 	} else if (fmt == 0x83F3u) {
-		a = blk[0];
-		const uint32_t a1 = blk[1];
-		uint64_t bits = 0;
-#pragma unroll
-		for (int q = 0; q < 6; ++q) bits |= (uint64_t)blk[2 + q] << (8 * q);
-		const int ab = (int)((bits >> (3 * ((j & 3) * 4 + (i & 3)))) & 7u);
-		if (ab == 1) a = a1;
-		else if (ab == 6 && a1 >= a) a = 0;
-		else if ((ab == 6 || ab == 7) && a1 >= a) a = 255;
-		else if (ab != 0 && odd) a = a1;
+		// the texel's 3-bit index from the two bytes that hold it (the
+		// second may be colour byte 8, whose bits are then not used)
+		const uint32_t bitpos = 3u * (uint32_t)((j & 3) * 4 + (i & 3));
+		const uint32_t q = bitpos >> 3, sh = bitpos & 7u;
+		const uint32_t two = (uint32_t)blk[2 + q] | ((uint32_t)blk[3 + q] << 8);
+		const uint32_t ab = (two >> sh) & 7u;
+		const uint32_t a0 = blk[0], a1 = blk[1];
+		const uint32_t six = a1 >= a0 ? 1u : 0u;  // 6-value mode: indices 6 and 7 are 0 and 255
+		// palette slot (0: a0, 1: a1, 2: 0, 3: 255) from (index, odd, six),
+		// then the byte of that slot: shifts only, no per-lane branches
+		const uint32_t slot = (uint32_t)(kAlphaSlots >> (2u * (ab * 4u + (odd ? 2u : 0u) + six))) & 3u;
+		a = ((a0 | (a1 << 8) | 0xff000000u) >> (8u * slot)) & 0xffu;
 	}
 	rgba[(size_t)j * (size_t)width + (size_t)i] = expand565(c) | (a << 24);
 }

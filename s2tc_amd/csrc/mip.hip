@@ -220,6 +220,6 @@ extern "C" int s2tc_amd_mip_chain_batch(const s2tc_amd_mip_job *jobs, int njobs,
 	}
 	// the task list is read by the kernels: keep it until they are done
 	const int rs = hip_check(hipStreamSynchronize(st), "mip_chain_batch synchronize");
-	if (d) hipFree(d);
+	if (d) (void)hipFree(d);
 	return rc ? rc : rs;
 }

@@ -16,6 +16,8 @@ Files written:
                 CLI's DDS files for the test images
   from_s3tc.npz DDS files of random S3TC blocks and the reference
                 s2tc_from_s3tc's output for each (oracle/_ref/s2tc_from_s3tc)
+  digests.json "decompress": SHA-256 of the reference s2tc_decompress's TGA
+                for the reference CLI's DDS of each test image and format
 """
 import argparse
 import hashlib
@@ -139,6 +141,27 @@ def cli_digests():
     return res
 
 
+def decompress_digests():
+    """TGA files of the reference decoder (oracle/_ref/s2tc_decompress with
+    the reference library) for the reference CLI's DDS files of the test
+    images, every format, setting faster-wavg-r."""
+    cli = os.path.join(ROOT, "oracle", "_ref", "s2tc_compress")
+    dec = os.path.join(ROOT, "oracle", "_ref", "s2tc_decompress")
+    lib = os.path.join(ROOT, "oracle", "_ref", "libs2tc_ref.so")
+    res = {}
+    tga_dir = os.path.join(HERE, "tga")
+    env = dict(os.environ, **workloads.CLI_SETTINGS["faster-wavg-r"])
+    for name in sorted(os.listdir(tga_dir)):
+        for fmt in ("DXT1", "DXT3", "DXT5"):
+            with tempfile.TemporaryDirectory() as td:
+                d, t = os.path.join(td, "o.dds"), os.path.join(td, "o.tga")
+                subprocess.run([cli, "-i", os.path.join(tga_dir, name), "-o", d, "-t", fmt, "-l", lib],
+                               check=True, env=env, capture_output=True)
+                subprocess.run([dec, "-i", d, "-o", t, "-l", lib], check=True, capture_output=True)
+                res[f"{name}/{fmt}"] = hashlib.sha256(open(t, "rb").read()).hexdigest()
+    return res
+
+
 def workload_digests(ref, skip_large):
     res = {}
     for name, wl in workloads.IMAGE_WORKLOADS.items():
@@ -239,6 +262,8 @@ def main():
         print("compress cases", compress_corpus(ref))
     if not only or "cli" in only:
         digests["cli"] = cli_digests()
+    if not only or "decompress" in only:
+        digests["decompress"] = decompress_digests()
     if not only or "from_s3tc" in only:
         print("from_s3tc cases", from_s3tc_corpus())
     if not only or "workloads" in only:

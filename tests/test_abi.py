@@ -109,3 +109,27 @@ print("ok")
     r = subprocess.run(["python", "-c", code], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     assert "libdxtn: Bad dstFormat 4660 in tx_compress_dxtn" in r.stderr
+
+
+def test_oracle_decode_image_against_compiled_reference(oracle, reference):
+    """The oracle's whole-image decode equals the compiled reference's
+    fetch_2d_texel_* texel by texel on a ragged image, with DXT5 blocks in the
+    6-value mode, the opaque (254, 255, all indices 7) block and DXT1 blocks
+    with c1 >= c0 among random ones."""
+    rng = np.random.default_rng(9)
+    W, H = 23, 14
+    for fmt, name in FETCH.items():
+        theirs = getattr(reference.lib, name)
+        theirs.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
+        bs = 8 if fmt < 0x83F2 else 16
+        nb = ((W + 3) // 4) * ((H + 3) // 4)
+        blob = rng.integers(0, 256, nb * bs).astype(np.uint8).reshape(nb, bs)
+        blob[1::3, 0] = 3
+        if fmt == 0x83F3:
+            blob[2::4, :8] = [254, 255, 255, 255, 255, 255, 255, 255]
+        got = oracle.decode_image(blob.reshape(-1), fmt, W, H).reshape(H, W, 4)
+        for j in range(H):
+            for i in range(W):
+                t = np.zeros(4, np.uint8)
+                theirs(W, blob.ctypes.data, i, j, t.ctypes.data)
+                assert np.array_equal(got[j, i], t), (fmt, i, j)

@@ -63,3 +63,44 @@ def test_colordist_error_rgba8_source(gpu, oracle):
         _say(f"case error rgba8 dxt={dxt} cd={cd}")
         assert gpu.colordist_error(rgba, w, h, blocks, dxt, cd, src_kind=gpu.SRC_RGBA8) == \
             oracle.colordist_error(q, w, h, dxt, cd, blocks)
+
+
+@pytest.mark.parametrize("fmt", FMTS)
+def test_decode_large_images(gpu, oracle, fmt):
+    """Whole images of several waves per row against the oracle's decode:
+    random blocks, DXT1 blocks with c1 >= c0 and DXT5 blocks in the 6-value
+    mode mixed in, among them the opaque block (254, 255, every index 7)."""
+    rng = np.random.default_rng(40 + fmt)
+    for w, h in [(1030, 518), (512, 512)]:
+        bw, bh = (w + 3) // 4, (h + 3) // 4
+        bs = 8 if fmt in (0x83F0, 0x83F1) else 16
+        blocks = rng.integers(0, 256, (bw * bh, bs)).astype(np.uint8)
+        blocks[1::3, 8 if bs == 16 else 0] = 3
+        blocks[2::5, 0] = 1
+        if fmt == 0x83F3:
+            blocks[3::4, :8] = [254, 255, 255, 255, 255, 255, 255, 255]
+        blocks = blocks.reshape(-1)
+        _say(f"case decode large fmt={fmt:#x} {w}x{h}")
+        exp = oracle.decode_image(blocks, fmt, w, h)
+        for rep in range(2):
+            got = gpu.decode_image(blocks, fmt, w, h)
+            bad = np.nonzero((got != exp).reshape(-1, 4).any(axis=1))[0]
+            assert bad.size == 0, (fmt, w, h, rep, bad.size, int(bad[0]) % w, int(bad[0]) // w)
+
+
+def test_decode_opaque_dxt5_encoding(gpu, oracle):
+    """An opaque image encoded as DXT5 (alpha 255 everywhere: endpoints
+    254, 255 and every index 7) decodes to alpha 255 at every texel, on
+    repeated decodes of the same blocks, and to the oracle's colours."""
+    rng = np.random.default_rng(77)
+    w = h = 1024
+    rgba = imgutil.smooth_rgba8(rng, w, h).reshape(h, w, 4).copy()
+    rgba[..., 3] = 255
+    _say(f"case decode opaque DXT5 {w}x{h}")
+    blocks = gpu.encode_image(rgba.reshape(-1), w, h, 2, 5, -1, 1, src_kind=gpu.SRC_RGBA8)
+    exp = oracle.decode_image(blocks, 0x83F3, w, h)
+    assert (exp.reshape(-1, 4)[:, 3] == 255).all()
+    for rep in range(3):
+        got = gpu.decode_image(blocks, 0x83F3, w, h)
+        bad = np.nonzero((got != exp).reshape(-1, 4).any(axis=1))[0]
+        assert bad.size == 0, (rep, bad.size, int(bad[0]) % w, int(bad[0]) // w)

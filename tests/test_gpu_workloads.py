@@ -108,3 +108,22 @@ def test_reference_cli_loads_this_library(gpu):
         _say(f"reference cli -l ours {key}")
         got = _run_cli(ref_cli, lib, os.path.join(GOLDEN, "tga", name), fmt, workloads.CLI_SETTINGS[setting])
         assert got == digest, key
+
+
+def test_decompress_cli_matches_reference_tga(gpu):
+    """This project's s2tc_decompress (GPU decode) on the DDS files this
+    project's CLI writes for the reference's test images: the TGA bytes the
+    reference s2tc_decompress writes for the reference CLI's DDS files."""
+    from s2tc_amd import workloads
+    cli = os.path.join(ROOT, "s2tc_amd", "s2tc_compress")
+    dec = os.path.join(ROOT, "s2tc_amd", "s2tc_decompress")
+    env = dict(os.environ, **workloads.CLI_SETTINGS["faster-wavg-r"])
+    with tempfile.TemporaryDirectory() as td:
+        for key, digest in sorted(DIGESTS["decompress"].items()):
+            name, fmt = key.split("/")
+            d, t = os.path.join(td, "o.dds"), os.path.join(td, "o.tga")
+            _say(f"decompress {key}")
+            subprocess.run([cli, "-i", os.path.join(GOLDEN, "tga", name), "-o", d, "-t", fmt], check=True, env=env,
+                           capture_output=True, timeout=300)
+            subprocess.run([dec, "-i", d, "-o", t], check=True, capture_output=True, timeout=300)
+            assert hashlib.sha256(open(t, "rb").read()).hexdigest() == digest, key
