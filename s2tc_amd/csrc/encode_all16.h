@@ -193,6 +193,9 @@ S2TC_D BlockOut encode_block_all16(const uint32_t (&tx)[16], int w, int h, int r
 		}
 	}
 	__builtin_amdgcn_sched_barrier(0);
+	// NORMALMAP DXT5: reload the texel features from the table here rather
+	// than keep the triangle's 48 VGPRs of them live across the alpha scan
+	if constexpr (!Dist<CD>::kCheap && DXT == DXT5) asm volatile("" ::: "memory");
 	TexFeat<CD> f;
 	f.init(tx);
 	const Masks mk = block_masks<DXT>(tx, w, h);

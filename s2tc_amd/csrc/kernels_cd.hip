@@ -147,9 +147,15 @@ __global__ __launch_bounds__(256) void k_fast_interior(const uint32_t *__restric
 	store_block<DXT>(dst, b, o);
 }
 
+// Waves per SIMD the all-16 kernels are compiled for.  Their pair scans
+// want more than 256 VGPRs; two waves (256 VGPRs, a few values in scratch)
+// hide memory latency better than one for every distance but the quadratic
+// ones, whose DXT5 forms lose more to the scratch traffic than they gain.
+template <int CD> struct All16Waves { static const int value = (CD == CD_AVG || CD == CD_WAVG || CD == CD_W0AVG) ? 1 : 2; };
+
 // MODE_NORMAL without random colours over one image: one lane per block.
 template <int CD, int DXT, bool RAW>
-__global__ __launch_bounds__(256) void k_all16(const uint32_t *__restrict__ src, int width, int height,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(All16Waves<CD>::value))) void k_all16(const uint32_t *__restrict__ src, int width, int height,
                                                uint8_t *__restrict__ dst, int refine, int vec, int by0)
 {
 	S2TC_IMG_BLOCK_COORDS
@@ -241,7 +247,7 @@ __global__ __launch_bounds__(256) void k_fast_batch(const JobDesc *__restrict__ 
 }
 
 template <int CD, int DXT, int REFINE>
-__global__ __launch_bounds__(256) void k_all16_batch(const JobDesc *__restrict__ jobs,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(All16Waves<CD>::value))) void k_all16_batch(const JobDesc *__restrict__ jobs,
                                                      const TileJobs *__restrict__ tile_jobs, long long nblocks,
                                                      int ntiles)
 {
