@@ -29,7 +29,7 @@
 
 namespace s2tc {
 
-#define S2TC_RS 20  // LDS row stride in ints (16 values + 4 pad: 16-byte aligned, fewer bank conflicts)
+#define S2TC_RS 16  // LDS row stride in ints: no padding, so that 7 four-block workgroups of m = 80 fit a CU
 
 // Every control used here (quad_perm, row mirrors, row_ror) reads a lane of
 // the same row, and rows are all-active or all-inactive, so bound_ctrl never
@@ -270,7 +270,9 @@ S2TC_D uint32_t reduce_rows(const int *rows, int t, int m, int pbw)
 	return st.lo;
 }
 
-// Per-group LDS: candidates (565 | alpha << 16), their features, the rows.
+// Per-group LDS: the rows, the features of the first 16 candidates (the
+// columns of every row), candidates (565 | alpha << 16).  A row's own
+// candidate feature is computed by its lane.
 template <int CD> struct GroupSmem {
 	uint32_t *cand;
 	typename Dist<CD>::F *feat;
@@ -279,7 +281,7 @@ template <int CD> struct GroupSmem {
 
 template <int CD> S2TC_D size_t group_smem_bytes(int mc)
 {
-	return (size_t)mc * (sizeof(uint32_t) + sizeof(typename Dist<CD>::F) + S2TC_RS * sizeof(int));
+	return (size_t)mc * (sizeof(uint32_t) + S2TC_RS * sizeof(int)) + 16 * sizeof(typename Dist<CD>::F);
 }
 
 template <int CD> S2TC_D GroupSmem<CD> group_smem(uint8_t *base, int g, int mc)
@@ -288,7 +290,7 @@ template <int CD> S2TC_D GroupSmem<CD> group_smem(uint8_t *base, int g, int mc)
 	GroupSmem<CD> s;
 	s.rows = (int *)p;
 	s.feat = (typename Dist<CD>::F *)(p + (size_t)mc * S2TC_RS * sizeof(int));
-	s.cand = (uint32_t *)(p + (size_t)mc * (S2TC_RS * sizeof(int) + sizeof(typename Dist<CD>::F)));
+	s.cand = (uint32_t *)(p + (size_t)mc * S2TC_RS * sizeof(int) + 16 * sizeof(typename Dist<CD>::F));
 	return s;
 }
 
@@ -485,16 +487,15 @@ S2TC_D void encode_block_normal(uint32_t tex, int t, int w, int h, int dxt, int 
 	}
 	__syncthreads();
 
-	// features of every candidate
-	for (int i = t; i < 16 * pbw; i += 16)
-		if (i < m) sm.feat[i] = D::feat(sm.cand[i] & 0xffffu);
+	// features of the first 16 candidates (the columns)
+	if (t < m) sm.feat[t] = D::feat(sm.cand[t] & 0xffffu);
 	__syncthreads();
 
 	// dists[i][k] rows (:373-392): square part dist(c[min], c[max]), box part dist(c[i], c[k])
 	for (int p = 0; p < pbw; ++p) {
 		const int i = 16 * p + t;
 		const bool irow = i < m;
-		F fi = irow ? sm.feat[i] : sm.feat[0];
+		const F fi = D::feat(sm.cand[irow ? i : 0] & 0xffffu);
 		int dv[16];
 #pragma unroll
 		for (int k = 0; k < 16; ++k) {

@@ -427,5 +427,6 @@ extern "C" void S2TC_CAT(s2tc_kernels_cd, S2TC_CD)(KernelTable *kt)
 #else
 	kt->init = nullptr;
 #endif
-	kt->normal_smem_per_cand = (int)(4 * (sizeof(uint32_t) + sizeof(Dist<S2TC_CD>::F) + S2TC_RS * sizeof(int)));
+	kt->normal_smem_per_cand = (int)(4 * (sizeof(uint32_t) + S2TC_RS * sizeof(int)));
+	kt->normal_smem_fixed = (int)(4 * 16 * sizeof(Dist<S2TC_CD>::F));
 }

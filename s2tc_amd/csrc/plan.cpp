@@ -81,7 +81,7 @@ int plan_classes(const s2tc_amd_job *jobs, int njobs, std::vector<ClassPlan> &cl
 		c.kernel = c.normal ? kt.normal_batch : c.all16 ? kt.all16_batch[c.dxt][c.refine] : kt.fast_batch[c.dxt][c.refine];
 		if (!c.kernel) return set_error(S2TC_E_ARG, "no batch kernel for distance %d", c.cd);
 		c.ntiles = (int)((c.nblocks + c.tile - 1) / c.tile);
-		c.smem = c.normal ? (size_t)kt.normal_smem_per_cand * 16 * c.pbw_max : 0;
+		c.smem = c.normal ? (size_t)kt.normal_smem_per_cand * 16 * c.pbw_max + (size_t)kt.normal_smem_fixed : 0;
 		// tile -> (first job, last job), by a walk over the packed blocks
 		c.tiles.resize((size_t)c.ntiles);
 		int jlo = 0;

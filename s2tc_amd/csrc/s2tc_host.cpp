@@ -156,7 +156,7 @@ int launch_encode(const EncodeArgs &a, hipStream_t stream)
 	int pbw = normal_pbw(nrandom);
 	unsigned long long seed = a.seed;
 	unsigned block_base = a.block_index_base;
-	size_t smem = (size_t)kt.normal_smem_per_cand * 16 * pbw;
+	size_t smem = (size_t)kt.normal_smem_per_cand * 16 * pbw + (size_t)kt.normal_smem_fixed;
 	void *args[] = {(void *)&src, (void *)&width, (void *)&height, (void *)&dst, (void *)&dxt,
 	                (void *)&nrandom, (void *)&refine, (void *)&seed, (void *)&pbw, (void *)&block_base};
 	dim3 grid((unsigned)((nb + 3) / 4)), block(64);
