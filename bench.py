@@ -493,8 +493,11 @@ def main():
                 c["ms"] = min(c["ms"], n["ms"])
     dom = max(classes, key=lambda c: c["ms"])
     names = s.COLORDIST_NAMES
+    # a batch of one all-16 class runs the two-waves-per-SIMD form of the
+    # kernel for every distance but AVG (4), WAVG (5), W0AVG (6)
+    alone = len(classes) == 1 and dom["normal"] == 2 and dom["cd"] not in (4, 5, 6)
     kname = {0: f"k_fast_batch<{dom['cd']}, {dom['dxt']}, {dom['refine']}>", 1: f"k_normal_batch<{dom['cd']}>",
-             2: f"k_all16_batch<{dom['cd']}, {dom['dxt']}, {dom['refine']}>"}[dom["normal"]]
+             2: f"k_all16_batch{'_alone' if alone else ''}<{dom['cd']}, {dom['dxt']}, {dom['refine']}>"}[dom["normal"]]
     gbs = dom["bytes"] / (dom["ms"] * 1e-3) / 1e9
     tr = measured_traffic().get(kname) if a.workload == "batch" else None
     roofline = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

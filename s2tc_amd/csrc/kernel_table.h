@@ -28,6 +28,7 @@ struct KernelTable {
 	const void *normal_batch;     // persistent MODE_NORMAL kernel over a job list
 	const void *all16[3][2];      // [dxt][raw]: MODE_NORMAL without random colours, one lane per block
 	const void *all16_batch[3][3];  // [dxt][refine]: its persistent batch form
+	const void *all16_batch_alone[3][3];  // the form for a batch of this one class
 	const void *error[3][2];      // [dxt][raw]: k_colordist_error (total colour distance of an encoding)
 	const void *init;             // per-device table set-up kernel (256 x 256 threads, one int argument), or null
 };

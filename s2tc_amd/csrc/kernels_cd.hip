@@ -147,10 +147,11 @@ __global__ __launch_bounds__(256) void k_fast_interior(const uint32_t *__restric
 	store_block<DXT>(dst, b, o);
 }
 
-// Waves per SIMD the all-16 kernels are compiled for.  Their pair scans
-// want more than 256 VGPRs; two waves (256 VGPRs, a few values in scratch)
-// hide memory latency better than one for every distance but the quadratic
-// ones, whose DXT5 forms lose more to the scratch traffic than they gain.
+// Waves per SIMD the one-image all-16 kernel is compiled for.  Its pair
+// scans want more than 256 VGPRs; two waves (256 VGPRs, some values in
+// scratch) hide memory latency better than one for every distance but the
+// quadratic ones, whose DXT5 forms lose more to the scratch traffic than
+// they gain.
 template <int CD> struct All16Waves { static const int value = (CD == CD_AVG || CD == CD_WAVG || CD == CD_W0AVG) ? 1 : 2; };
 
 // MODE_NORMAL without random colours over one image: one lane per block.
@@ -247,9 +248,8 @@ __global__ __launch_bounds__(256) void k_fast_batch(const JobDesc *__restrict__ 
 }
 
 template <int CD, int DXT, int REFINE>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(All16Waves<CD>::value))) void k_all16_batch(const JobDesc *__restrict__ jobs,
-                                                     const TileJobs *__restrict__ tile_jobs, long long nblocks,
-                                                     int ntiles)
+S2TC_D void all16_batch_tiles(const JobDesc *__restrict__ jobs, const TileJobs *__restrict__ tile_jobs,
+                              long long nblocks, int ntiles)
 {
 	for (int tile = (int)blockIdx.x; tile < ntiles; tile += (int)gridDim.x) {
 		const long long gb = (long long)tile * 256 + threadIdx.x;
@@ -267,6 +267,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(All16Waves<
 		const BlockOut o = encode_block_all16<CD, DXT>(tx, w, h, REFINE);
 		store_block<DXT>(j.dst, (size_t)b, o);
 	}
+}
+
+// One wave per SIMD: inside a mixed batch the other classes fill the issue
+// slots this kernel leaves idle, and at 256 VGPRs its DXT5 forms spill to
+// scratch, whose write-backs multiplied its HBM writes tenfold for no gain.
+template <int CD, int DXT, int REFINE>
+__global__ __launch_bounds__(256) void k_all16_batch(const JobDesc *__restrict__ jobs,
+                                                     const TileJobs *__restrict__ tile_jobs, long long nblocks,
+                                                     int ntiles)
+{
+	all16_batch_tiles<CD, DXT, REFINE>(jobs, tile_jobs, nblocks, ntiles);
+}
+
+// The same for a class that has the GPU to itself (a batch of one kernel
+// class, such as one large image): nothing else fills the idle issue slots,
+// and two waves per SIMD are faster despite the scratch traffic.
+template <int CD, int DXT, int REFINE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_all16_batch_alone(
+    const JobDesc *__restrict__ jobs, const TileJobs *__restrict__ tile_jobs, long long nblocks, int ntiles)
+{
+	all16_batch_tiles<CD, DXT, REFINE>(jobs, tile_jobs, nblocks, ntiles);
 }
 
 template <int CD>
@@ -419,6 +440,16 @@ extern "C" void S2TC_CAT(s2tc_kernels_cd, S2TC_CD)(KernelTable *kt)
 	kt->all16_batch[D][1] = (const void *)k_all16_batch<S2TC_CD, D, REFINE_ALWAYS>; \
 	kt->all16_batch[D][2] = (const void *)k_all16_batch<S2TC_CD, D, REFINE_LOOP>;
 	S2TC_AB3(DXT1) S2TC_AB3(DXT3) S2TC_AB3(DXT5)
+	for (int d = 0; d < 3; ++d)
+		for (int r = 0; r < 3; ++r) kt->all16_batch_alone[d][r] = kt->all16_batch[d][r];
+	if (All16Waves<S2TC_CD>::value == 2) {
+#define S2TC_AL3(D)                                                                        \
+	kt->all16_batch_alone[D][0] = (const void *)k_all16_batch_alone<S2TC_CD, D, REFINE_NEVER>;  \
+	kt->all16_batch_alone[D][1] = (const void *)k_all16_batch_alone<S2TC_CD, D, REFINE_ALWAYS>; \
+	kt->all16_batch_alone[D][2] = (const void *)k_all16_batch_alone<S2TC_CD, D, REFINE_LOOP>;
+		S2TC_AL3(DXT1) S2TC_AL3(DXT3) S2TC_AL3(DXT5)
+#undef S2TC_AL3
+	}
 #undef S2TC_AB3
 	kt->normal[0] = (const void *)k_normal<S2TC_CD, false>;
 	kt->normal[1] = (const void *)k_normal<S2TC_CD, true>;

@@ -94,6 +94,9 @@ int plan_classes(const s2tc_amd_job *jobs, int njobs, std::vector<ClassPlan> &cl
 			c.tiles[(size_t)t] = TileJobs{jlo, jhi};
 		}
 	}
+	// a lone all-16 class has the GPU to itself: its two-waves-per-SIMD form
+	if (classes.size() == 1 && classes[0].all16)
+		classes[0].kernel = kernel_table(classes[0].cd).all16_batch_alone[classes[0].dxt][classes[0].refine];
 	std::stable_sort(classes.begin(), classes.end(),
 	                 [](const ClassPlan &x, const ClassPlan &y) { return x.cost > y.cost; });
 	return 0;

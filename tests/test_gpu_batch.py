@@ -118,3 +118,30 @@ def test_full_batch_matches_reference_digest(gpu):
     assert hashlib.sha256(hashes).hexdigest() == exp["sha256_of_job_sha256s"]
     b.destroy()
     imgs.free()
+
+
+@pytest.mark.parametrize("cd", [0, 1, 2, 3, 7, 5])
+def test_single_class_batch(gpu, oracle, cd):
+    """A batch whose images all fall into one all-16 class (nrandom 0, one
+    format and refinement) runs that class's two-waves-per-SIMD kernel form;
+    every output equals the oracle, for every format and refinement."""
+    rng = np.random.default_rng(900 + cd)
+    for dxt in range(3):
+        for refine in range(3):
+            items = []
+            for w, h in [(37, 23), (64, 64), (8, 4), (1, 2)]:
+                items.append(dict(rgba=rng.integers(0, 256, w * h * 4).astype(np.uint8), width=w, height=h,
+                                  dxt=dxt, cd=cd, nrandom=0, refine=refine, seed=0))
+            _say(f"single-class batch cd={cd} dxt={dxt} refine={refine}")
+            imgs = gpu.DeviceImages(items)
+            b = gpu.Batch(imgs)
+            b.set_schedule(4, 2)
+            b.launch()
+            gpu.synchronize()
+            host = imgs.dst.to_host()
+            for i, it in enumerate(items):
+                q = oracle.rgb565_image(it["rgba"], it["width"], it["height"], 4, {0: 1, 1: 4, 2: 8}[dxt], 0)
+                exp = oracle.encode_quantized(q, it["width"], it["height"], dxt, cd, 0, refine, 0)
+                assert np.array_equal(imgs.output(i, host), exp), (cd, dxt, refine, i)
+            b.destroy()
+            imgs.free()
