@@ -133,3 +133,26 @@ def test_oracle_decode_image_against_compiled_reference(oracle, reference):
                 t = np.zeros(4, np.uint8)
                 theirs(W, blob.ctypes.data, i, j, t.ctypes.data)
                 assert np.array_equal(got[j, i], t), (fmt, i, j)
+
+
+@pytest.mark.parametrize("w,h", [(0, 5), (7, 0), (0, 0), (-3, 4)])
+def test_empty_images_write_nothing(reference, w, h):
+    """Width or height of zero (or below): tx_compress_dxtn and rgb565_image
+    write nothing and return, as the reference's block loops do (compared
+    against the compiled reference); no device work is started."""
+    import s2tc_amd
+    src = np.arange(64, dtype=np.uint8)
+    for lib in (s2tc_amd.lib(), reference.lib):
+        f = lib.tx_compress_dxtn
+        f.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint, ctypes.c_void_p,
+                      ctypes.c_int]
+        for fmt in (0x83F0, 0x83F1, 0x83F2, 0x83F3):
+            dst = np.full(64, 0xA5, np.uint8)
+            f(4, w, h, src.ctypes.data, fmt, dst.ctypes.data, 16)
+            assert (dst == 0xA5).all(), (lib, fmt, w, h)
+        g = lib.rgb565_image
+        g.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                      ctypes.c_int]
+        out = np.full(64, 0x5A, np.uint8)
+        g(out.ctypes.data, src.ctypes.data, w, h, 4, 8, 1)
+        assert (out == 0x5A).all(), (lib, w, h)
