@@ -41,6 +41,22 @@ constexpr uint64_t alpha_slots()
 }
 constexpr uint64_t kAlphaSlots = alpha_slots();
 
+// Colour slot for (index b, texel odd, DXT1 block with c1 >= c0) at bit
+// 2 * (b * 4 + odd * 2 + black): 0 = c0, 1 = c1, 2 = black.
+constexpr uint32_t colour_slots()
+{
+	uint32_t t = 0;
+	for (unsigned b = 0; b < 4; ++b)
+		for (unsigned odd = 0; odd < 2; ++odd)
+			for (unsigned black = 0; black < 2; ++black) {
+				unsigned slot = b == 0 ? 0u : b == 1 ? 1u : odd ? 1u : 0u;
+				if (black && b == 3) slot = 2;
+				t |= slot << (2 * (b * 4 + odd * 2 + black));
+			}
+	return t;
+}
+constexpr uint32_t kColourSlots = colour_slots();
+
 __global__ __launch_bounds__(256) void k_decode(const uint8_t *__restrict__ blocks, unsigned fmt, int width,
                                                 int height, uint32_t *__restrict__ rgba)
 {
@@ -49,19 +65,16 @@ __global__ __launch_bounds__(256) void k_decode(const uint8_t *__restrict__ bloc
 	const int bpb = (fmt == 0x83F0u || fmt == 0x83F1u) ? 8 : 16;
 	const uint8_t *blk = blocks + ((size_t)((width + 3) >> 2) * (size_t)(j >> 2) + (size_t)(i >> 2)) * bpb;
 	const uint8_t *cb = bpb == 8 ? blk : blk + 8;
-	uint32_t c = cb[0] | (cb[1] << 8);
+	const uint32_t c0 = cb[0] | (cb[1] << 8);
 	const uint32_t c1 = cb[2] | (cb[3] << 8);
-	const int b = (cb[4 + (j & 3)] >> (2 * (i & 3))) & 3;
-	uint32_t a = 255;
+	const uint32_t b = (uint32_t)(cb[4 + (j & 3)] >> (2 * (i & 3))) & 3u;
 	const bool odd = ((i ^ j) & 1) != 0;
-	if (b == 1) {
-		c = c1;
-	} else if (bpb == 8 && b == 3 && c1 >= c) {
-		c = 0;
-		if (fmt == 0x83F1u) a = 0;
-	} else if (b >= 2 && odd) {
-		c = c1;
-	}
+	// colour slot (0: c0, 1: c1, 2: black) from (index, odd, DXT1 with
+	// c1 >= c0), then the 16-bit field of that slot: shifts only
+	const uint32_t black = (bpb == 8 && c1 >= c0) ? 1u : 0u;
+	const uint32_t slot = (uint32_t)(kColourSlots >> (2u * (b * 4u + (odd ? 2u : 0u) + black))) & 3u;
+	const uint32_t c = (uint32_t)(((uint64_t)c0 | ((uint64_t)c1 << 16)) >> (16u * slot)) & 0xffffu;
+	uint32_t a = (fmt == 0x83F1u && slot == 2u) ? 0u : 255u;
 	if (fmt == 0x83F2u) {
 		const uint32_t q = (blk[(j & 3) * 2 + ((i & 3) >> 1)] >> (4 * (i & 1))) & 15u;
 		a = q | (q << 4);
