@@ -104,9 +104,9 @@ S2TC_D uint32_t sign_bytes(int v0, int v1, int v2, int v3)
 	return ((lo | hi) >> 7) & 0x01010101u;
 }
 
-// 0/1 bytes x = 0..3 -> bits 2x (one multiply: byte x lands on bit 24 + 2x and
-// no other partial product reaches bits 24..31)
-S2TC_D uint32_t bytes_to_slots(uint32_t m) { return (m * 0x01041040u) >> 24; }
+// 0/1 bytes x = 0..3 -> bits 2x: one dot product with the byte weights 1, 4,
+// 16, 64 (a full-rate v_dot4 where a 32-bit multiply is quarter rate)
+S2TC_D uint32_t bytes_to_slots(uint32_t m) { return __builtin_amdgcn_udot4(m, 0x40100401u, 0u, false); }
 
 struct QSums {
 	int r, g, b;  // sums of the dot bytes (r, AG*g, b)
