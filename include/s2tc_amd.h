@@ -40,9 +40,11 @@ extern "C" {
 #define S2TC_E_BUDGET   (-5)   /* request exceeds the serving memory budget */
 
 /* largest nrandom the GPU path supports: the candidate list (16 + nrandom
- * colours, rounded up to 16) and its distance rows for four blocks must fit
- * the 64 KiB of LDS a workgroup can allocate */
-#define S2TC_MAX_NRANDOM 144
+ * colours, rounded up to 16, 4 bytes each), its distance rows (64 bytes
+ * each) and the features of the first 16 candidates, for four blocks, must
+ * fit the 64 KiB of LDS a workgroup can allocate: 224 candidates take
+ * 61,696 bytes, 240 would take 66,048 */
+#define S2TC_MAX_NRANDOM 208
 
 /* source kinds for s2tc_amd_encode_image */
 #define S2TC_SRC_QUANTIZED 0   /* rgb565_image output: r5 g6 b5 a, 4 bytes per texel */

@@ -187,4 +187,4 @@ def test_nrandom_limit_is_an_error(gpu):
     src = gpu.DeviceBuffer(64)
     dst = gpu.DeviceBuffer(16)
     with pytest.raises(gpu.S2tcError, match="nrandom"):
-        gpu.encode_image_device(src, 0, 4, 4, dst, 0, 5, 145, 1)
+        gpu.encode_image_device(src, 0, 4, 4, dst, 0, 5, 209, 1)

@@ -139,12 +139,12 @@ def test_uniform_alpha_blocks(gpu, oracle, raw):
 def test_random_candidate_counts(gpu, oracle, cd):
     """Random-candidate blocks with candidate counts 16 + nrandom that are not
     multiples of 16 (odd nrandom; DXT1 blocks whose transparent texels leave
-    fewer than 16 colours), up to the GPU limit of 144, every format: the
+    fewer than 16 colours), up to the GPU limit of 208, every format: the
     bounded distances (AVG, WAVG, W0AVG, NORMALMAP) and DXT5 alpha take the
     keyed pair scan, SRGB the per-step scan with its negative-sum rule."""
     rng = np.random.default_rng(300 + cd)
     for dxt in range(3):
-        for nr in (1, 15, 17, 31, 48, 144):
+        for nr in (1, 15, 17, 31, 48, 144, 208):
             w, h = (9, 6) if nr < 100 else (5, 4)
             for kind in ("speckle", "random", "extreme"):
                 q = imgutil.random_quantized(rng, w, h, dxt, kind)
