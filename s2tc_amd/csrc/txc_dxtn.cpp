@@ -9,11 +9,15 @@
 // different threads do not share buffers.
 #include "s2tc_internal.h"
 
+#include <algorithm>
+#include <atomic>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <strings.h>
+#include <thread>
 #include <vector>
 
 #include "s2tc_algorithm.h"
@@ -57,24 +61,79 @@ struct HostBuf {
 };
 
 // Inputs up to this size are copied into pinned staging before the upload;
-// larger ones go to the runtime's pageable copy path.
+// larger ones stream through two pinned buffers of this size.
 static const size_t kStageMax = 8u << 20;
+// Host threads (the caller's included) that fill a staging buffer.
+static const unsigned kUploadThreads = 4;
 
 struct Ctx {
 	hipStream_t stream = nullptr;
 	DevBuf src, q, out, scratch;
-	HostBuf in_stage, host_out;
+	HostBuf in_stage, in_stage2, host_out;
+	hipEvent_t stage_ev[2] = {nullptr, nullptr};  // last DMA out of in_stage / in_stage2
 };
+
+// A large upload in chunks of kStageMax bytes through the two pinned
+// buffers: while the DMA of chunk i runs, the host threads copy chunk i + 1
+// into the other buffer (one host thread alone copies pageable memory well
+// below the link rate).  A buffer is refilled only after the event of the
+// DMA that last read it.
+static int upload_large(Ctx *c, void *dst, const void *src, size_t n)
+{
+	int rc = c->in_stage.ensure(kStageMax);
+	if (!rc) rc = c->in_stage2.ensure(kStageMax);
+	for (int b = 0; b < 2 && !rc; ++b)
+		if (!c->stage_ev[b])
+			rc = hip_check(hipEventCreateWithFlags(&c->stage_ev[b], hipEventDisableTiming), "hipEventCreate");
+	if (rc) return rc;
+	const long nchunks = (long)((n + kStageMax - 1) / kStageMax);
+	const unsigned hw = std::thread::hardware_concurrency();
+	const unsigned nt = std::max(1u, std::min(kUploadThreads, hw ? hw : 1u));
+	std::unique_ptr<std::atomic<unsigned>[]> done(new std::atomic<unsigned>[(size_t)nchunks]);
+	for (long i = 0; i < nchunks; ++i) done[i].store(0);
+	std::atomic<long> open{-1};  // the highest chunk the threads may copy
+	std::atomic<bool> abort{false};
+	uint8_t *bufs[2] = {c->in_stage.data(), c->in_stage2.data()};
+	auto copy_slice = [&](long i, unsigned t) {
+		const size_t off = (size_t)i * kStageMax, len = std::min(kStageMax, n - off);
+		const size_t per = (len / nt + 63) & ~(size_t)63, lo = std::min(len, per * t), hi = std::min(len, lo + per);
+		if (hi > lo) memcpy(bufs[i & 1] + lo, (const uint8_t *)src + off + lo, hi - lo);
+		done[i].fetch_add(1, std::memory_order_release);
+	};
+	std::vector<std::thread> workers;
+	for (unsigned t = 1; t < nt; ++t)
+		workers.emplace_back([&, t] {
+			for (long i = 0; i < nchunks; ++i) {
+				while (open.load(std::memory_order_acquire) < i) {
+					if (abort.load(std::memory_order_acquire)) return;
+					std::this_thread::yield();
+				}
+				if (abort.load(std::memory_order_acquire)) return;
+				copy_slice(i, t);
+			}
+		});
+	for (long i = 0; i < nchunks && !rc; ++i) {
+		rc = hip_check(hipEventSynchronize(c->stage_ev[i & 1]), "hipEventSynchronize");
+		if (rc) break;
+		open.store(i, std::memory_order_release);
+		copy_slice(i, 0);
+		while (done[i].load(std::memory_order_acquire) < nt) std::this_thread::yield();
+		const size_t off = (size_t)i * kStageMax, len = std::min(kStageMax, n - off);
+		rc = hip_check(hipMemcpyAsync((uint8_t *)dst + off, bufs[i & 1], len, hipMemcpyHostToDevice, c->stream), "H2D");
+		if (!rc) rc = hip_check(hipEventRecord(c->stage_ev[i & 1], c->stream), "hipEventRecord");
+	}
+	if (rc) abort.store(true, std::memory_order_release);
+	for (auto &w : workers) w.join();
+	return rc;
+}
 
 // host -> device upload of a caller's buffer on the context's stream
 static int upload(Ctx *c, void *dst, const void *src, size_t n)
 {
-	if (n <= kStageMax) {
-		if (int r = c->in_stage.ensure(n)) return r;
-		memcpy(c->in_stage.p, src, n);
-		src = c->in_stage.p;
-	}
-	return hip_check(hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, c->stream), "H2D");
+	if (n > kStageMax) return upload_large(c, dst, src, n);
+	if (int r = c->in_stage.ensure(n)) return r;
+	memcpy(c->in_stage.p, src, n);
+	return hip_check(hipMemcpyAsync(dst, c->in_stage.p, n, hipMemcpyHostToDevice, c->stream), "H2D");
 }
 
 static std::mutex g_pool_mu;
