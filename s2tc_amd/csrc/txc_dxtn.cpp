@@ -361,11 +361,28 @@ void tx_compress_dxtn(GLint srccomps, GLint width, GLint height, const GLubyte *
 	const int bs = block_bytes(dxt);
 	const int bw = (width + 3) / 4, bh = (height + 3) / 4;
 	const long rowdiff = dstRowStride >= width * k ? (long)dstRowStride - (long)((width + 3) & ~3) * k : 0;
-	GLubyte *blk = dest;
-	for (int by = 0; by < bh; ++by) {
-		memcpy(blk, g.c->host_out.data() + (size_t)by * bw * bs, (size_t)bw * bs);
-		blk += (long)bw * bs + rowdiff;
+	const size_t rowbytes = (size_t)bw * bs;
+	const uint8_t *hout = g.c->host_out.data();
+	auto copy_rows = [&](int lo, int hi) {
+		for (int by = lo; by < hi; ++by)
+			memcpy(dest + ((long)rowbytes + rowdiff) * by, hout + (size_t)by * rowbytes, rowbytes);
+	};
+	const unsigned hw = std::thread::hardware_concurrency();
+	const unsigned nt = std::max(1u, std::min(kUploadThreads, hw ? hw : 1u));
+	if (rowdiff < 0 || nt == 1 || rowbytes * (size_t)bh < (2u << 20)) {
+		// overlapping rows (a negative row gap) must be written in order
+		copy_rows(0, bh);
+		return;
 	}
+	// disjoint rows: several host threads, contiguous row ranges each
+	std::vector<std::thread> workers;
+	const int per = (bh + (int)nt - 1) / (int)nt;
+	for (unsigned t = 1; t < nt; ++t) {
+		const int lo = std::min(bh, per * (int)t), hi = std::min(bh, lo + per);
+		if (hi > lo) workers.emplace_back(copy_rows, lo, hi);
+	}
+	copy_rows(0, std::min(bh, per));
+	for (auto &w : workers) w.join();
 }
 
 void rgb565_image(unsigned char *out, const unsigned char *rgba, int w, int h, int srccomps, int alphabits,

@@ -188,3 +188,31 @@ def test_nrandom_limit_is_an_error(gpu):
     dst = gpu.DeviceBuffer(16)
     with pytest.raises(gpu.S2tcError, match="nrandom"):
         gpu.encode_image_device(src, 0, 4, 4, dst, 0, 5, 209, 1)
+
+
+def test_large_call_staging_and_row_gaps(gpu, clean_env):
+    """Sources above the single staging buffer stream through chunked pinned
+    buffers, and large outputs are laid out by several host threads: a
+    2048x1024 DXT5 call (8 MiB in, 2 MiB of blocks out, 3 MiB with the row
+    gap) equals the same call with exact rows placed with the reference's
+    dstRowDiff rule, and nothing outside the rows is written."""
+    L = gpu.lib()
+    L.tx_compress_dxtn.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint,
+                                   ctypes.c_void_p, ctypes.c_int]
+    os.environ.update(S2TC_DITHER_MODE="NONE", S2TC_COLORDIST_MODE="WAVG", S2TC_RANDOM_COLORS="-1",
+                      S2TC_REFINE_COLORS="ALWAYS")
+    rng = np.random.default_rng(12)
+    w, h = 2048, 1024 + 3  # a partial last block row
+    src = rng.integers(0, 256, w * h * 4).astype(np.uint8)
+    bw, bh, bs = w // 4, (h + 3) // 4, 16
+    _say(f"large tx_compress_dxtn {w}x{h}, exact rows")
+    exact = np.zeros(bw * bh * bs, np.uint8)
+    L.tx_compress_dxtn(4, w, h, src.ctypes.data, 0x83F3, exact.ctypes.data, bw * bs)
+    pad = 2048  # dstRowStride - ((w + 3) & ~3) * 4
+    stride = bw * bs + pad
+    _say(f"large tx_compress_dxtn {w}x{h}, row stride {stride}")
+    dst = np.full(stride * bh, 0x5C, np.uint8)
+    L.tx_compress_dxtn(4, w, h, src.ctypes.data, 0x83F3, dst.ctypes.data, stride)
+    rows = dst.reshape(bh, stride)
+    assert np.array_equal(rows[:, :bw * bs].reshape(-1), exact)
+    assert (rows[:, bw * bs:] == 0x5C).all()
