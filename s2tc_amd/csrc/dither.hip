@@ -389,7 +389,9 @@ static int launch_rgb565_fs(const uint8_t *src, int srccomps, int w, int h, int 
 // super chunk (16..256).
 static void simple_chunking(long long n, DChans &cs)
 {
-	int L = 16;
+	// chunks of at least 32 texels: on a 256x256 image 16-texel chunks made
+	// the call 4% slower (more maps to compose, no shorter serial walk)
+	int L = 32;
 	while (L < 256 && (long long)L * 16384 < n) L <<= 1;
 	const long long nchunks = (n + L - 1) / L;
 	int K = 16;
