@@ -59,51 +59,151 @@ def image_items(name):
                  nrandom=wl["nrandom"], refine=wl["refine"], seed=wl["seed"])]
 
 
-def cpu_baseline(items, budget_s=12.0, imgs=None):
-    """The compiled reference (oracle/_ref), threads over all host cores, on a
-    sample of the workload's jobs: jobs are taken in a fixed stride order
-    until about budget_s of single-thread-equivalent work is reached."""
+def cpu_model():
+    """(CPU model name from /proc/cpuinfo, host cores this process may use)."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    ncores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return model, ncores
+
+
+def median_of_5(fn):
+    """One untimed warm-up call, then the median wall time of 5 calls."""
+    import numpy as np
+    fn()
+    ts = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts))
+
+
+def _ref_job_type():
+    class RefJob(ctypes.Structure):
+        _fields_ = [("q", ctypes.c_void_p), ("dest", ctypes.c_void_p), ("width", ctypes.c_int),
+                    ("height", ctypes.c_int), ("dxt", ctypes.c_int), ("cd", ctypes.c_int),
+                    ("nrandom", ctypes.c_int), ("refine", ctypes.c_int), ("seed", ctypes.c_uint64)]
+    return RefJob
+
+
+def cpu_baseline(items, imgs=None, stride=5):
+    """The compiled reference (oracle/_ref: the unmodified encoder sources),
+    threads over all host cores, on every stride-th job of the workload (a
+    fixed 1-in-`stride` sample: one pass over the full 16467-job list takes
+    about 11 s on 16 cores, and 1 warm-up + 5 timed passes of it would hold
+    the GPU box for over a minute).  Quantization (DITHER_NONE, as the GPU
+    path) is done before timing; the timed region is the reference's block
+    loop only.  1 warm-up pass, median of 5."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
     from oracle import Reference, reference_available
     if not reference_available():
         return None
     ref = Reference()
-
-    class RefJob(ctypes.Structure):
-        _fields_ = [("q", ctypes.c_void_p), ("dest", ctypes.c_void_p), ("width", ctypes.c_int),
-                    ("height", ctypes.c_int), ("dxt", ctypes.c_int), ("cd", ctypes.c_int),
-                    ("nrandom", ctypes.c_int), ("refine", ctypes.c_int), ("seed", ctypes.c_uint64)]
+    RefJob = _ref_job_type()
     ref.lib.ref_encode_jobs_mt.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
     ref.lib.ref_encode_jobs_mt.restype = ctypes.c_int
-    ncores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
-    # stride sample over the jobs, grown until the sample is big enough
-    order = list(range(0, len(items), 7)) + [i for i in range(len(items)) if i % 7]
-    keep, arr, texels, t_used = [], [], 0, 0.0
-    chunk = max(1, len(items) // 50)
-    pos = 0
-    while pos < len(order) and t_used < budget_s / 2:
-        sel = order[pos:pos + chunk]
-        pos += chunk
-        batch, bkeep = [], []
-        for i in sel:
-            it = items[i]
-            w, h = it["width"], it["height"]
-            rgba = it["rgba"] if it["rgba"] is not None else imgs.source(i)
-            q = ref.rgb565_image(rgba, w, h, 4, {0: 1, 1: 4, 2: 8}[it["dxt"]], 0)
-            dst = np.zeros(((w + 3) // 4) * ((h + 3) // 4) * (8 if it["dxt"] == 0 else 16), np.uint8)
-            bkeep += [q, dst]
-            batch.append(RefJob(q.ctypes.data, dst.ctypes.data, w, h, it["dxt"], it["cd"], it["nrandom"],
-                                it["refine"], it["seed"]))
-            texels += w * h
-        ja = (RefJob * len(batch))(*batch)
-        t0 = time.perf_counter()
-        n = ref.lib.ref_encode_jobs_mt(ctypes.cast(ja, ctypes.c_void_p), len(batch), ncores)
-        t_used += time.perf_counter() - t0
-        keep += bkeep
-    return dict(value=texels / t_used / 1e6, unit="Mtexels/s", cores=int(n), kind="reference",
-                sample=f"{texels} texels ({pos} of {len(items)} images, every 7th first) of the same workload, "
-                       f"DITHER_NONE quantization + the reference's block loop, {n} threads, {t_used:.1f} s")
+    model, ncores = cpu_model()
+    keep, batch, texels = [], [], 0
+    for i in range(0, len(items), stride):
+        it = items[i]
+        w, h = it["width"], it["height"]
+        rgba = it["rgba"] if it["rgba"] is not None else imgs.source(i)
+        q = ref.rgb565_image(rgba, w, h, 4, {0: 1, 1: 4, 2: 8}[it["dxt"]], 0)
+        dst = np.zeros(((w + 3) // 4) * ((h + 3) // 4) * (8 if it["dxt"] == 0 else 16), np.uint8)
+        keep += [q, dst]
+        batch.append(RefJob(q.ctypes.data, dst.ctypes.data, w, h, it["dxt"], it["cd"], it["nrandom"],
+                            it["refine"], it["seed"]))
+        texels += w * h
+    ja = (RefJob * len(batch))(*batch)
+    used = [0]
+
+    def run():
+        used[0] = ref.lib.ref_encode_jobs_mt(ctypes.cast(ja, ctypes.c_void_p), len(batch), ncores)
+    t = median_of_5(run)
+    return dict(value=texels / t / 1e6, unit="Mtexels/s", cores=int(used[0]), kind="reference", cpu_model=model,
+                sample=f"every {stride}th job of the same workload: {len(batch)} of {len(items)} images, "
+                       f"{texels} texels; DITHER_NONE quantization before timing, the reference's block loop "
+                       f"timed, {used[0]} threads, 1 warm-up + median of 5 passes ({t:.2f} s each)")
+
+
+def image_configs_leg(s, with_cpu=True, reps=5):
+    """The two single-image configurations, device-resident, next to the
+    compiled reference on the host cores:
+      tiles4096  4096x4096 RGBA8 tiles, DXT5, SRGB_MIXED, all 16 candidates, LOOP
+      normal8192 8192x8192 RGB8 normal map, DXT1, NORMALMAP, nrandom 64, LOOP
+    Both quantized with the library default SIMPLE dither first (as
+    tx_compress_dxtn does; not timed).  GPU: encode kernels on the quantized
+    image in HBM, HIP events, median of `reps` launches after one warm-up.
+    CPU: the reference's block loop over a band of the same quantized image
+    (the top 1024 block rows' texels of tiles4096, 64 texel rows of
+    normal8192: the whole images would take 2 s and 93 s per pass), all host
+    cores, 1 warm-up + median of 5."""
+    import numpy as np
+    from s2tc_amd import workloads
+    ref = None
+    if with_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        from oracle import Reference, reference_available
+        ref = Reference() if reference_available() else None
+    model, ncores = cpu_model()
+    ev = HipEvents()
+    out = []
+    for name, band in (("tiles4096", 1024), ("normal8192", 64)):
+        wl = workloads.IMAGE_WORKLOADS[name]
+        w, h = wl["width"], wl["height"]
+        src = workloads.make_image_source(wl)
+        q = s.rgb565_image(src, w, h, wl["srccomps"], {0: 1, 1: 4, 2: 8}[wl["dxt"]], wl["dither"])
+        del src
+        d_q = s.DeviceBuffer.from_host(q)
+        nb = s.nblocks(w, h)
+        d_o = s.DeviceBuffer(nb * s.block_bytes(wl["dxt"]))
+        args = (d_q, s.SRC_QUANTIZED, w, h, d_o, wl["dxt"], wl["cd"], wl["nrandom"], wl["refine"], wl["seed"])
+        s.encode_image_device(*args)
+        s.synchronize()
+        ts = []
+        for _ in range(reps):
+            ev.record(0)
+            s.encode_image_device(*args)
+            ev.record(1)
+            ts.append(ev.elapsed_ms())
+        ms = float(np.median(ts))
+        m = 16 + max(0, wl["nrandom"])
+        scans = nb
+        if wl["dxt"] == 2:
+            # the DXT5 alpha scan runs for every block but the full ones with one alpha value
+            fw, fh = w // 4, h // 4
+            al = q.reshape(h, w, 4)[:4 * fh, :4 * fw, 3].reshape(fh, 4, fw, 4)
+            scans += nb - int((al == al[:, :1, :, :1]).all(axis=(1, 3)).sum())
+        ops = 16 * m * (m - 1) * scans
+        row = dict(workload=name, texels=w * h, kernel_ms=round(ms, 4), mtexels_s=round(w * h / (ms * 1e-3) / 1e6, 1),
+                   valu_floor_ops=ops, valu_frac=round(ops / (ms * 1e-3) / 1e12 / VALU_PEAK_TOPS, 4))
+        if ref is not None:
+            bh = min(h, band)
+            qb = np.ascontiguousarray(q[:w * bh * 4])
+            used = [0]
+
+            def run():
+                used[0] = ref.encode_quantized_mt(qb, w, bh, wl["dxt"], wl["cd"], wl["nrandom"], wl["refine"],
+                                                  seed=wl["seed"], nthreads=ncores)[1]
+            t = median_of_5(run)
+            row["cpu_reference"] = dict(value=round(w * bh / t / 1e6, 2), unit="Mtexels/s", cores=int(used[0]),
+                                        cpu_model=model, sample=f"top {bh} of {h} texel rows ({w * bh} texels), "
+                                                                f"1 warm-up + median of 5")
+        log(f"config {row}")
+        out.append(row)
+        d_q.free()
+        d_o.free()
+    ev.close()
+    return out
 
 
 def alpha_scan_blocks(imgs, cd, refine):
@@ -151,39 +251,47 @@ class HipEvents:
             self.h.hipEventDestroy(e)
 
 
-def quick_leg(s, size=16384, reps=10):
-    """Quick mode (MODE_FAST) on one large RGBA8 image in HBM: the sinusoid +
-    noise content of the 256x256 case at size x size, WAVG, REFINE_ALWAYS,
-    DITHER_NONE quantization on load; k_fast launches timed with HIP events.
+def quick_leg(s, size=16384, reps=10, cds=(0, 1, 2, 3, 4, 5, 6)):
+    """Quick mode (MODE_FAST) on large RGBA8 images in HBM, every distance
+    that has a quick mode (all but NORMALMAP), REFINE_ALWAYS, DITHER_NONE
+    quantization on load; k_fast_interior launches timed with HIP events.
+    DXT1 on the sinusoid + noise content of the 256x256 case at size x size
+    (opaque), DXT5 on the gradient-tiles content (radial alpha with 0/255
+    speckle: the alpha endpoint search runs on almost every block).
     Algorithmic bytes: 4 per texel read + 8 (DXT1) or 16 (DXT5) per block
     written."""
     from s2tc_amd import synth
-    src = synth.sinus(size, size, 0)
-    d_src = s.DeviceBuffer.from_host(src)
-    del src
+    names = s.COLORDIST_NAMES
     ev = HipEvents()
     out = []
-    for dxt in (0, 2):
+    traffic = measured_traffic()
+    for dxt, gen in ((0, synth.sinus), (2, synth.tiles)):
+        src = gen(size, size, 0)
+        d_src = s.DeviceBuffer.from_host(src)
+        del src
         nb = s.nblocks(size, size)
         d_dst = s.DeviceBuffer(nb * s.block_bytes(dxt))
-        s.encode_image_device(d_src, s.SRC_RGBA8, size, size, d_dst, dxt, s.WAVG, -1, s.REFINE_ALWAYS)
-        s.synchronize()
-        ev.record(0)
-        for _ in range(reps):
-            s.encode_image_device(d_src, s.SRC_RGBA8, size, size, d_dst, dxt, s.WAVG, -1, s.REFINE_ALWAYS)
-        ev.record(1)
-        ms = ev.elapsed_ms() / reps
-        texels = size * size
-        nbytes = 4 * texels + nb * s.block_bytes(dxt)
-        gbs = nbytes / (ms * 1e-3) / 1e9
-        kname = f"k_fast_interior<5, {dxt}, true, 1>"
-        tr = measured_traffic().get(kname) if size == 16384 else None
-        out.append(dict(kernel=kname, image=f"{size}x{size}", dxt=(1, 3, 5)[dxt], texels=texels,
-                        kernel_ms=round(ms, 4), mtexels_s=round(texels / (ms * 1e-3) / 1e6, 1),
-                        algorithmic_bytes=nbytes, achieved_gbs=round(gbs, 1), frac=round(gbs / HBM_PEAK_GBS, 4),
-                        traffic=sum(tr) if tr else None))
-        log(f"quick {out[-1]}")
+        for cd in cds:
+            s.encode_image_device(d_src, s.SRC_RGBA8, size, size, d_dst, dxt, cd, -1, s.REFINE_ALWAYS)
+            s.synchronize()
+            ev.record(0)
+            for _ in range(reps):
+                s.encode_image_device(d_src, s.SRC_RGBA8, size, size, d_dst, dxt, cd, -1, s.REFINE_ALWAYS)
+            ev.record(1)
+            ms = ev.elapsed_ms() / reps
+            texels = size * size
+            nbytes = 4 * texels + nb * s.block_bytes(dxt)
+            gbs = nbytes / (ms * 1e-3) / 1e9
+            kname = f"k_fast_interior<{cd}, {dxt}, true, 1>"
+            tr = traffic.get(kname) if size == 16384 else None
+            out.append(dict(kernel=kname, cd=names[cd], image=f"{size}x{size} {gen.__name__}", dxt=(1, 3, 5)[dxt],
+                            texels=texels, kernel_ms=round(ms, 4), mtexels_s=round(texels / (ms * 1e-3) / 1e6, 1),
+                            algorithmic_bytes=nbytes, achieved_gbs=round(gbs, 1), frac=round(gbs / HBM_PEAK_GBS, 4),
+                            traffic=sum(tr) if tr else None))
+            log(f"quick {out[-1]}")
         d_dst.free()
+        if dxt == 0:
+            d_src.free()
     # a plain streaming copy of the same source, for the achievable read+write rate
     d_cp = s.DeviceBuffer(d_src.nbytes)
     L = s.lib()
@@ -389,6 +497,51 @@ def serve_leg(s, rates, duration, npool=256, seed=4):
     return out
 
 
+def verify_batch_output(imgs, workload):
+    """SHA-256 over the per-image block digests of the batch output, against
+    the compiled reference's digest of the same jobs (tests/golden/
+    digests.json).  True / False, or None where no digest exists."""
+    import hashlib
+    digests = json.load(open(os.path.join(ROOT, "tests", "golden", "digests.json")))["workloads"]
+    exp = digests.get(workload) if workload == "batch" else None
+    if not exp or len(imgs.items) != exp["jobs"]:
+        return None
+    host = imgs.dst.to_host()
+    hashes = b"".join(hashlib.sha256(imgs.output(i, host).tobytes()).digest() for i in range(len(imgs.items)))
+    return hashlib.sha256(hashes).hexdigest() == exp["sha256_of_job_sha256s"]
+
+
+def class_roofline(s, c, imgs, nclasses):
+    """Adds to a class_times entry: its kernel's name, HBM rate and fraction
+    (algorithmic bytes: 4 per texel read + the block bytes written) and, for
+    the all-16 and random-candidate classes, the pair-scan floor of
+    reduce_colors_inplace - n m (m - 1) min+add int32 lane-ops per block with
+    n = 16 texel colours and m = 16 + nrandom candidates, plus the same again
+    for the DXT5 alpha scan of every block that runs one (full blocks with a
+    single alpha value run none) - and its fraction of the VALU peak."""
+    # a batch of one all-16 class runs the two-waves-per-SIMD form of the
+    # kernel for every distance but AVG (4), WAVG (5), W0AVG (6)
+    alone = nclasses == 1 and c["normal"] == 2 and c["cd"] not in (4, 5, 6)
+    c["kernel_name"] = {0: f"k_fast_batch<{c['cd']}, {c['dxt']}, {c['refine']}>", 1: f"k_normal_batch<{c['cd']}>",
+                        2: f"k_all16_batch{'_alone' if alone else ''}<{c['cd']}, {c['dxt']}, {c['refine']}>"}[c["normal"]]
+    c["gbs"] = c["bytes"] / (c["ms"] * 1e-3) / 1e9
+    c["hbm_frac"] = c["gbs"] / HBM_PEAK_GBS
+    ops = 0
+    if c["normal"] == 2:
+        scans = c["blocks"]
+        if c["dxt"] == 2:
+            scans += alpha_scan_blocks(imgs, c["cd"], c["refine"])
+        ops = 16 * 16 * 15 * scans
+    elif c["normal"] == 1:
+        for it in imgs.items:
+            if it["cd"] == c["cd"] and it["nrandom"] > 0:
+                m = 16 + it["nrandom"]
+                ops += s.nblocks(it["width"], it["height"]) * 16 * m * (m - 1) * (2 if it["dxt"] == 2 else 1)
+    c["valu_ops"] = ops
+    c["valu_tops"] = ops / (c["ms"] * 1e-3) / 1e12
+    c["valu_frac"] = c["valu_tops"] / VALU_PEAK_TOPS
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -403,6 +556,10 @@ def main():
     ap.add_argument("--quick-size", type=int, default=16384)
     ap.add_argument("--no-side", action="store_true")
     ap.add_argument("--no-calls", action="store_true")
+    ap.add_argument("--no-configs", action="store_true")
+    ap.add_argument("--profile-classes", action="store_true",
+                    help="only launch each kernel class alone, --steps times (give --schedule to skip the "
+                         "autotuner's graph launches), and print the class times")
     ap.add_argument("--only-calls", action="store_true",
                     help="only time tx_compress_dxtn per call on the single-image workloads and print their JSON")
     ap.add_argument("--calls-workloads", default="sinus256,tiles4096,normal8192")
@@ -472,15 +629,35 @@ def main():
     sched = batch.schedule()
     log(f"schedule wg_per_cu={sched[0]} streams={sched[1]} (autotuned {tuned_ms} ms)")
 
+    if a.profile_classes:
+        # standalone launches of each class only (no graph replays), so that a
+        # kernel trace's per-kernel averages are the class times printed here
+        for _ in range(a.steps):
+            ct = batch.class_times()
+        for c in ct:
+            class_roofline(s, c, imgs, len(ct))
+        print(json.dumps({"classes": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in c.items()}
+                                      for c in ct]}), flush=True)
+        batch.destroy()
+        imgs.free()
+        return
+
     for _ in range(a.warmup):
         batch.launch()
     s.synchronize()
+    imgs.dst.fill(0)   # the output checked below is written by the timed launches alone
     t0 = time.perf_counter()
     for _ in range(a.steps):
         batch.launch()
     s.synchronize()
     dt = (time.perf_counter() - t0) / a.steps
     value = texels / dt / 1e6
+    bit_exact = verify_batch_output(imgs, a.workload)
+    log(f"batch output after the timed launches equals the reference digest: {bit_exact}")
+    if bit_exact is False:
+        print(json.dumps({"error": "the timed batch output differs from the reference digest",
+                          "workload": a.workload}), flush=True)
+        sys.exit(3)
 
     # per-class kernel times (HIP events on the launch stream), repeated
     classes = None
@@ -491,46 +668,28 @@ def main():
         else:
             for c, n in zip(classes, ct):
                 c["ms"] = min(c["ms"], n["ms"])
-    dom = max(classes, key=lambda c: c["ms"])
     names = s.COLORDIST_NAMES
-    # a batch of one all-16 class runs the two-waves-per-SIMD form of the
-    # kernel for every distance but AVG (4), WAVG (5), W0AVG (6)
-    alone = len(classes) == 1 and dom["normal"] == 2 and dom["cd"] not in (4, 5, 6)
-    kname = {0: f"k_fast_batch<{dom['cd']}, {dom['dxt']}, {dom['refine']}>", 1: f"k_normal_batch<{dom['cd']}>",
-             2: f"k_all16_batch{'_alone' if alone else ''}<{dom['cd']}, {dom['dxt']}, {dom['refine']}>"}[dom["normal"]]
-    gbs = dom["bytes"] / (dom["ms"] * 1e-3) / 1e9
+    for c in classes:
+        class_roofline(s, c, imgs, len(classes))
+    dom = max(classes, key=lambda c: c["ms"])
+    kname = dom["kernel_name"]
     tr = measured_traffic().get(kname) if a.workload == "batch" else None
-    roofline = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": sum(tr) if tr else None, "kernel": kname,
-                "kernel_ms": round(dom["ms"], 4), "algorithmic_bytes": dom["bytes"]}
+    hbm = {"achieved": round(dom["gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(dom["hbm_frac"], 4),
+           "algorithmic_bytes": dom["bytes"]}
+    if dom["valu_ops"]:
+        # all-16 and random-candidate classes: bound by VALU issue
+        roofline = {"bound": "valu", "achieved": round(dom["valu_tops"], 3), "peak": VALU_PEAK_TOPS,
+                    "unit": "TOP/s (int32 lane-ops)", "frac": round(dom["valu_frac"], 4),
+                    "algorithmic_ops": dom["valu_ops"], "hbm": hbm}
+    else:
+        roofline = dict(bound="hbm", **hbm)
+    roofline.update(traffic=sum(tr) if tr else None, kernel=kname, kernel_ms=round(dom["ms"], 4))
     if tr:
         roofline["traffic_source"] = "profiles/traffic_per_launch.txt (rocprofv3 FETCH_SIZE x2, WRITE_SIZE x1)"
-    if dom["normal"] == 2:
-        # pair-scan floor of reduce_colors_inplace: n m (m - 1) min+add lane-ops
-        # per block with n = m = 16, and the same again for the DXT5 alpha
-        # scan of every block that needs one (full blocks with a single alpha
-        # need none)
-        scans = dom["blocks"]
-        if dom["dxt"] == 2:
-            scans += alpha_scan_blocks(imgs, dom["cd"], dom["refine"])
-        ops = 16 * 16 * 15 * scans
-    elif dom["normal"] == 1:
-        # the same floor with m = 16 + nrandom candidates for every job of the
-        # class (its distance, nrandom > 0), doubled for DXT5 alpha
-        ops = 0
-        for it in imgs.items:
-            if it["cd"] == dom["cd"] and it["nrandom"] > 0:
-                m = 16 + it["nrandom"]
-                ops += s.nblocks(it["width"], it["height"]) * 16 * m * (m - 1) * (2 if it["dxt"] == 2 else 1)
-    if dom["normal"] in (1, 2):
-        tops = ops / (dom["ms"] * 1e-3) / 1e12
-        roofline["valu"] = {"bound": "valu", "achieved": round(tops, 3), "peak": VALU_PEAK_TOPS,
-                            "unit": "Tlane-op/s", "frac": round(tops / VALU_PEAK_TOPS, 4),
-                            "algorithmic_ops": ops}
-    class_summary = [dict(kernel={0: "fast", 1: "normal", 2: "all16"}[c["normal"]], cd=names[c["cd"]], dxt=c["dxt"],
-                          refine=c["refine"],
-                          jobs=c["njobs"], texels=c["texels"], ms=round(c["ms"], 4),
-                          gbs=round(c["bytes"] / (c["ms"] * 1e-3) / 1e9, 1)) for c in classes]
+    class_summary = [dict(kernel=c["kernel_name"], cd=names[c["cd"]], dxt=c["dxt"], refine=c["refine"],
+                          jobs=c["njobs"], texels=c["texels"], ms=round(c["ms"], 4), gbs=round(c["gbs"], 1),
+                          hbm_frac=round(c["hbm_frac"], 4),
+                          valu_frac=round(c["valu_frac"], 4) if c["valu_ops"] else None) for c in classes]
 
     plain = None
     if not a.no_plain:
@@ -561,6 +720,10 @@ def main():
     if not a.no_serve and a.workload == "batch":
         serve = serve_leg(s, (2000, 10000, 50000), a.serve_duration)
 
+    configs = None
+    if not a.no_configs and a.workload == "batch":
+        configs = image_configs_leg(s, not a.no_cpu)
+
     cpu = None
     quality = None
     if not a.no_cpu:
@@ -579,6 +742,7 @@ def main():
                    "batch: 2000 RGBA8 textures 4..2048 per side + full mip chains (16467 images), "
                    "DXT1/3/5 30/30/40%, 8 distance modes, quick selection, REFINE_ALWAYS, DITHER_NONE",
                    "images": len(items), "texels": texels, "schedule": {"wg_per_cu": sched[0], "streams": sched[1]}},
+        "bit_exact": bit_exact,
         "plain_per_texture": plain,
         "device_mips": mip_leg,
         "roofline": roofline,
@@ -587,6 +751,7 @@ def main():
         "side_kernels": side,
         "calls": calls,
         "cpu_baseline": cpu,
+        "image_configs": configs,
         "serve": serve,
         "quality": quality,
     }

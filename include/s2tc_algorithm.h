@@ -15,7 +15,7 @@
  *   For nrandom > 0 each call draws from the counter-based generator keyed
  *   by (S2TC_RANDOM_SEED, per-thread block counter); the counter starts at 0
  *   in every thread and advances by one per call, and
- *   s2tc_amd_set_block_index() sets it.
+ *   s2tc_amd_set_block_index() (s2tc_amd.h) sets it.
  */
 #ifndef S2TC_COMPRESSOR_H
 #define S2TC_COMPRESSOR_H
@@ -46,10 +46,6 @@ typedef void (*s2tc_encode_block_func_t)(unsigned char *out, const unsigned char
                                          int nrandom);
 S2TC_API s2tc_encode_block_func_t s2tc_encode_block_func(enum DxtMode dxt, ColorDistMode cd, int nrandom,
                                                 enum RefinementMode refine);
-
-/* Sets the calling thread's block counter used as the generator key of the
- * next s2tc_encode_block_func call (no counterpart in the reference). */
-S2TC_API void s2tc_amd_set_block_index(unsigned int index);
 
 #ifdef __cplusplus
 }

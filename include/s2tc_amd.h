@@ -53,6 +53,12 @@ extern "C" {
 S2TC_API const char *s2tc_amd_last_error(void);
 S2TC_API int s2tc_amd_device_count(void);
 
+/* Sets the calling thread's block counter: the generator key (with
+ * S2TC_RANDOM_SEED) of the next per-block encoder call made through a
+ * s2tc_encode_block_func pointer (s2tc_algorithm.h).  The counter starts at
+ * 0 in every thread and advances by one per call. */
+S2TC_API void s2tc_amd_set_block_index(unsigned int index);
+
 /* device memory and stream plumbing (HIP device 0 unless set otherwise) */
 S2TC_API int s2tc_amd_malloc(void **dptr, size_t bytes);
 S2TC_API int s2tc_amd_free(void *dptr);

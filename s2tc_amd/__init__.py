@@ -145,6 +145,18 @@ def _dptr(b):
     return b.ptr if isinstance(b, DeviceBuffer) else b
 
 
+def rgb565_image(rgba, width, height, srccomps, alphabits, dither):
+    """rgb565_image of the C ABI (quantization and dither on the GPU): 8-bit
+    RGB(A) host bytes -> width*height quantized texels {r5, g6, b5, a}."""
+    L = lib()
+    L.rgb565_image.argtypes = [_P, _P, _I, _I, _I, _I, _I]
+    rgba = np.ascontiguousarray(rgba, dtype=np.uint8).reshape(-1)
+    assert rgba.size >= width * height * (3 if srccomps == 3 else 4)
+    out = np.zeros(width * height * 4, np.uint8)
+    L.rgb565_image(out.ctypes.data_as(_P), rgba.ctypes.data_as(_P), width, height, srccomps, alphabits, dither)
+    return out
+
+
 def decode_image_device(d_blocks, fmt, width, height, d_rgba, stream=None):
     """fetch_2d_texel_* of every texel into d_rgba (width*height RGBA8), asynchronous."""
     check(lib().s2tc_amd_decode_image(_dptr(d_blocks), fmt, width, height, _dptr(d_rgba), stream))

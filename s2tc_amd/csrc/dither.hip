@@ -387,7 +387,12 @@ static int launch_rgb565_fs(const uint8_t *src, int srccomps, int w, int h, int 
 // Chunk length L and super chunk size K for n texels: powers of two with
 // about n / 16384 texels per chunk (16..256) and about sqrt(chunks) chunks per
 // super chunk (16..256).
-static void simple_chunking(long long n, DChans &cs)
+struct SimpleChunking {
+	int L, K;
+	long long nchunks, nsuper, stride;
+};
+
+static constexpr SimpleChunking simple_chunking_of(long long n)
 {
 	// chunks of at least 32 texels: on a 256x256 image 16-texel chunks made
 	// the call 4% slower (more maps to compose, no shorter serial walk)
@@ -396,37 +401,48 @@ static void simple_chunking(long long n, DChans &cs)
 	const long long nchunks = (n + L - 1) / L;
 	int K = 16;
 	while (K < 256 && (long long)K * K < nchunks) K <<= 1;
-	cs.L = L;
-	cs.K = K;
-	cs.nchunks = nchunks;
-	cs.nsuper = (nchunks + K - 1) / K;
+	const long long nsuper = (nchunks + K - 1) / K;
 	// 4-byte aligned regions (the kernels copy maps as words)
-	cs.stride = ((nchunks + cs.nsuper) * DITHER_S + cs.nsuper + 255) & ~255LL;
+	return SimpleChunking{L, K, nchunks, nsuper, ((nchunks + nsuper) * DITHER_S + nsuper + 255) & ~255LL};
 }
+
+static void simple_chunking(long long n, DChans &cs)
+{
+	const SimpleChunking c = simple_chunking_of(n);
+	cs.L = c.L;
+	cs.K = c.K;
+	cs.nchunks = c.nchunks;
+	cs.nsuper = c.nsuper;
+	cs.stride = c.stride;
+}
+
+// scratch bytes of one dither mode: SIMPLE maps of up to four channels, FS
+// carry rows of three channels plus the row handed to the alpha pass
+static constexpr size_t simple_scratch_bytes(long long n)
+{
+	return (size_t)(4 * simple_chunking_of(n > 0 ? n : 1).stride + 256);
+}
+static constexpr size_t fs_scratch_bytes(int w) { return (size_t)(4 * ((long long)w + 2)) * sizeof(int) + 256; }
 
 size_t rgb565_scratch_bytes(int w, int h)
 {
-	const long long n = (long long)w * h;
-	DChans cs;
-	simple_chunking(n > 0 ? n : 1, cs);
-	const size_t simple = (size_t)(4 * cs.stride + 256);
-	const size_t fs = (size_t)(4 * (w + 2)) * sizeof(int) + 256;
+	const size_t simple = simple_scratch_bytes((long long)w * h), fs = fs_scratch_bytes(w);
 	return simple > fs ? simple : fs;
 }
 
-static int run_simple(const uint8_t *src, int srccomps, long long n, DChans cs, uint8_t *out, uint8_t *scratch,
+static int run_simple(const uint8_t *src, int srccomps, long long n, DChans cs, uint8_t *out, uint8_t *scr,
                       hipStream_t st)
 {
 	simple_chunking(n, cs);
 	const unsigned nch = (unsigned)cs.nch;
 	k_dither_chunk_maps<<<dim3((unsigned)((cs.nchunks * DITHER_S + 255) / 256), nch), 256, 0, st>>>(src, srccomps, n,
-	                                                                                                  cs, scratch);
-	k_dither_super_maps<<<dim3((unsigned)((cs.nsuper + 3) / 4), nch), 128, (size_t)4 * cs.K * DITHER_S, st>>>(cs, scratch);
+	                                                                                                  cs, scr);
+	k_dither_super_maps<<<dim3((unsigned)((cs.nsuper + 3) / 4), nch), 128, (size_t)4 * cs.K * DITHER_S, st>>>(cs, scr);
 	const size_t lds = (size_t)(cs.nsuper * DITHER_S);
 	const int staged = lds <= 60000;
-	k_dither_super_starts<<<dim3(1, nch), 256, staged ? lds : 0, st>>>(cs, scratch, staged);
+	k_dither_super_starts<<<dim3(1, nch), 256, staged ? lds : 0, st>>>(cs, scr, staged);
 	k_dither_emit<<<dim3((unsigned)cs.nsuper, nch), (unsigned)cs.K, (size_t)cs.K * DITHER_S, st>>>(src, srccomps, n, cs,
-	                                                                                             scratch, out);
+	                                                                                             scr, out);
 	return hip_check(hipGetLastError(), "dither SIMPLE kernels");
 }
 

@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "s2tc_algorithm.h"
+#include "staging.h"
 #include "txc_dxtn.h"
 
 namespace s2tc {
@@ -96,7 +97,8 @@ static int upload_large(Ctx *c, void *dst, const void *src, size_t n)
 	uint8_t *bufs[2] = {c->in_stage.data(), c->in_stage2.data()};
 	auto copy_slice = [&](long i, unsigned t) {
 		const size_t off = (size_t)i * kStageMax, len = std::min(kStageMax, n - off);
-		const size_t per = (len / nt + 63) & ~(size_t)63, lo = std::min(len, per * t), hi = std::min(len, lo + per);
+		size_t lo, hi;
+		stage_slice(len, nt, t, lo, hi);
 		if (hi > lo) memcpy(bufs[i & 1] + lo, (const uint8_t *)src + off + lo, hi - lo);
 		done[i].fetch_add(1, std::memory_order_release);
 	};

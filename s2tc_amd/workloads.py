@@ -38,6 +38,7 @@ CLI_SETTINGS = {
 BATCH_SEED = 3
 BATCH_COUNT = 2000
 SERVE_SEED = 4
+SERVE_POOL = 256
 
 ALPHABITS = {0: 1, 1: 4, 2: 8}
 DITHER_NONE, DITHER_SIMPLE = 0, 1

@@ -18,6 +18,8 @@ Files written:
                 s2tc_from_s3tc's output for each (oracle/_ref/s2tc_from_s3tc)
   digests.json "decompress": SHA-256 of the reference s2tc_decompress's TGA
                 for the reference CLI's DDS of each test image and format
+  digests.json "workloads"/"serve_pool": per-texture SHA-256 of the serving
+                pool's encodings (every mip level), one per serving request
 """
 import argparse
 import hashlib
@@ -182,9 +184,9 @@ def workload_digests(ref, skip_large):
     return res
 
 
-def batch_digest_reference(ref, jobs):
-    """Reference encode of every (texture, level) job, DITHER_NONE quantization;
-    digest = SHA-256 over the concatenated per-job block digests."""
+def reference_texture_blocks(ref, jobs, chunk=400):
+    """Reference encode of every (texture, level) job, DITHER_NONE
+    quantization; yields (texture index, [blocks of each level])."""
     import ctypes
 
     class RefJob(ctypes.Structure):
@@ -192,26 +194,47 @@ def batch_digest_reference(ref, jobs):
                     ("height", ctypes.c_int), ("dxt", ctypes.c_int), ("cd", ctypes.c_int),
                     ("nrandom", ctypes.c_int), ("refine", ctypes.c_int), ("seed", ctypes.c_uint64)]
     ref.lib.ref_encode_jobs_mt.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
-    hashes = []
-    CH = 400
-    for t0 in range(0, len(jobs.textures), CH):
-        texs = jobs.textures[t0:t0 + CH]
-        keep = []
-        arr = []
+    for t0 in range(0, len(jobs.textures), chunk):
+        texs = jobs.textures[t0:t0 + chunk]
+        keep, arr, outs = [], [], []
         for tex in texs:
+            lv_out = []
             for lv in tex["levels"]:
                 w, h = lv["width"], lv["height"]
                 q = ref.rgb565_image(lv["rgba"], w, h, 4, {0: 1, 1: 4, 2: 8}[tex["dxt"]], 0)
                 dst = np.zeros(((w + 3) // 4) * ((h + 3) // 4) * (8 if tex["dxt"] == 0 else 16), np.uint8)
-                keep += [q, dst]
+                keep.append(q)
+                lv_out.append(dst)
                 arr.append(RefJob(q.ctypes.data, dst.ctypes.data, w, h, tex["dxt"], tex["cd"], tex["nrandom"],
                                   tex["refine"], lv["seed"]))
+            outs.append(lv_out)
         ja = (RefJob * len(arr))(*arr)
         ref.lib.ref_encode_jobs_mt(ctypes.cast(ja, ctypes.c_void_p), len(arr), 0)
-        for i in range(1, len(keep), 2):
-            hashes.append(hashlib.sha256(keep[i].tobytes()).digest())
-        print(f"batch textures {t0 + len(texs)}/{len(jobs.textures)}", flush=True)
+        for k, lv_out in enumerate(outs):
+            yield t0 + k, lv_out
+        print(f"textures {t0 + len(texs)}/{len(jobs.textures)}", flush=True)
+
+
+def batch_digest_reference(ref, jobs):
+    """Digest of the batch = SHA-256 over the concatenated per-job block digests."""
+    hashes = []
+    for _, lv_out in reference_texture_blocks(ref, jobs):
+        hashes += [hashlib.sha256(b.tobytes()).digest() for b in lv_out]
     return {"jobs": len(hashes), "texels": jobs.texels, "sha256_of_job_sha256s": hashlib.sha256(b"".join(hashes)).hexdigest()}
+
+
+def serve_pool_digests(ref):
+    """The serving pool (256 textures of the serving mix: 70% quick mode, 30%
+    nrandom 64 + LOOP, seed SERVE_SEED) with full mip chains: per texture -
+    one serving request - the SHA-256 of its levels' blocks concatenated,
+    base level first."""
+    jobs = workloads.batch_jobs(workloads.SERVE_SEED, workloads.SERVE_POOL, serve_mix=True)
+    per_tex = [None] * len(jobs.textures)
+    njobs = 0
+    for ti, lv_out in reference_texture_blocks(ref, jobs, chunk=64):
+        per_tex[ti] = hashlib.sha256(b"".join(b.tobytes() for b in lv_out)).hexdigest()
+        njobs += len(lv_out)
+    return {"textures": len(per_tex), "jobs": njobs, "texels": jobs.texels, "texture_sha256": per_tex}
 
 
 def s3tc_dds(rng, fourcc, nblocks, bs, tail=3):
@@ -268,6 +291,8 @@ def main():
         print("from_s3tc cases", from_s3tc_corpus())
     if not only or "workloads" in only:
         digests.setdefault("workloads", {}).update(workload_digests(ref, a.skip_large))
+    if not only or "serve" in only:
+        digests.setdefault("workloads", {})["serve_pool"] = serve_pool_digests(ref)
     json.dump(digests, open(dig_path, "w"), indent=1, sort_keys=True)
 
 

@@ -190,6 +190,26 @@ def test_nrandom_limit_is_an_error(gpu):
         gpu.encode_image_device(src, 0, 4, 4, dst, 0, 5, 209, 1)
 
 
+@pytest.mark.parametrize("w,h,sc", [(2049, 2049, 3), (2049, 1025, 4), (1531, 1373, 4)])
+def test_large_call_ragged_chunks_against_oracle(gpu, oracle, clean_env, w, h, sc):
+    """Sources above 8 MiB whose last staging chunk is not a multiple of the
+    host threads' 64-byte slices (2049 x 2049 RGB: a last chunk of 4,206,595
+    bytes; 2049 x 1025 and 1531 x 1373 RGBA) equal the oracle's
+    tx_compress_dxtn byte for byte, the last texels included."""
+    L = gpu.lib()
+    L.tx_compress_dxtn.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint,
+                                   ctypes.c_void_p, ctypes.c_int]
+    rng = np.random.default_rng(w * 7 + h + sc)
+    src = rng.integers(0, 256, w * h * sc).astype(np.uint8)
+    bw, bh = (w + 3) // 4, (h + 3) // 4
+    for fmt, bs in ((0x83F0, 8), (0x83F3, 16)):
+        _say(f"large tx_compress_dxtn {w}x{h} srccomps={sc} format={fmt:#x}, default settings")
+        dst = np.zeros(bw * bh * bs, np.uint8)
+        L.tx_compress_dxtn(sc, w, h, src.ctypes.data, fmt, dst.ctypes.data, bw * bs)
+        r, exp = oracle.compress(sc, w, h, src, fmt, bw * bs, 5, -1, 1, 1)
+        assert np.array_equal(dst, exp[:dst.size]), (w, h, sc, fmt)
+
+
 def test_large_call_staging_and_row_gaps(gpu, clean_env):
     """Sources above the single staging buffer stream through chunked pinned
     buffers, and large outputs are laid out by several host threads: a

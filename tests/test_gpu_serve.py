@@ -1,6 +1,12 @@
-"""The serving loop: every request's blocks are the same alone and in the
-mix; budget admission queues or rejects instead of failing."""
+"""The serving loop: every request's blocks equal the compiled reference's
+(per-texture digests of the serving pool, tests/golden/digests.json) under
+Poisson arrivals, are the same alone and in the mix, and budget admission
+queues or rejects instead of failing."""
+import hashlib
+import json
+import os
 import sys
+import types
 
 import numpy as np
 import pytest
@@ -80,3 +86,73 @@ def test_serve_budget_queues_and_rejects(gpu):
     assert stats["completed"] + stats["rejected"] == len(reqs)
     assert ((lat < 0) == np.array([n > small for n in need])).all()
     imgs.free()
+
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.fixture(scope="module")
+def serve_pool(gpu):
+    """The serving pool on the device: 256 textures of the serving mix (70%
+    quick mode, 30% nrandom 64 + LOOP), base levels uploaded and every mip
+    level built on the device, as the benchmark's serving leg does."""
+    from s2tc_amd import workloads
+    exp = json.load(open(os.path.join(ROOT, "tests", "golden", "digests.json")))["workloads"]["serve_pool"]
+    _say("building the serving pool (256 textures, device mips)")
+    textures = workloads.batch_textures(workloads.SERVE_SEED, workloads.SERVE_POOL, serve_mix=True)
+    imgs = gpu.DeviceImages.from_textures(textures)
+    groups, f = [], 0
+    for t in textures:
+        n = max(t["width"], t["height"]).bit_length()   # levels down to 1x1
+        groups.append((f, n))
+        f += n
+    assert f == len(imgs.items) == exp["jobs"] and imgs.texels == exp["texels"]
+    yield imgs, groups, exp["texture_sha256"]
+    imgs.free()
+
+
+def _poisson_requests(gpu, imgs, groups, rate, duration, seed):
+    """Requests drawn from the pool with exponential inter-arrival times; each
+    request gets its own copies of its texture's jobs (same device sources),
+    so every request's blocks come back to a buffer of their own."""
+    rng = np.random.default_rng(seed)
+    n = max(1, int(rate * duration))
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n))
+    picks = rng.integers(0, len(groups), n)
+    total = sum(groups[p][1] for p in picks)
+    jobs = (gpu.Job * total)()
+    reqs, outs, k = [], [], 0
+    for a, p in zip(arrivals, picks):
+        f, m = groups[p]
+        for q in range(m):
+            j = imgs.jobs[f + q]
+            jobs[k + q] = j
+            outs.append(np.zeros(gpu.nblocks(j.width, j.height) * gpu.block_bytes(j.dxt), np.uint8))
+        reqs.append((float(a), k, m))
+        k += m
+    return types.SimpleNamespace(jobs=jobs), reqs, outs, picks
+
+
+@pytest.mark.parametrize("rate,duration,serial", [(10000, 0.2, False), (50000, 0.05, False), (2000, 0.1, True)])
+def test_serve_poisson_matches_reference_digests(gpu, serve_pool, rate, duration, serial):
+    """A Poisson stream from the serving pool through s2tc_amd_serve (10k
+    req/s, the benchmark's middle rate; 50k req/s, where requests queue and
+    batches are large; serial plain launches at 2k req/s): every request's
+    blocks, all mip levels, hash to the compiled reference's digest of its
+    texture."""
+    imgs, groups, digests = serve_pool
+    view, reqs, outs, picks = _poisson_requests(gpu, imgs, groups, rate, duration, seed=rate + int(serial))
+    _say(f"serve {len(reqs)} Poisson requests at {rate}/s serial={serial} ({len(outs)} images)")
+    stats, lat = gpu.serve(view, reqs, hbm_budget=64 << 30, streams=4, max_batch=256, serial_plain=serial,
+                           keep_outputs=outs)
+    _say(f"served: p50 {stats['p50_ms']:.3f} ms p99 {stats['p99_ms']:.3f} ms batches {stats['batches']}")
+    assert stats["completed"] == len(reqs) and stats["rejected"] == 0
+    assert (lat > 0).all()
+    bad = []
+    for r, ((_, first, m), p) in enumerate(zip(reqs, picks)):
+        h = hashlib.sha256(b"".join(outs[first + q].tobytes() for q in range(m))).hexdigest()
+        if h != digests[p]:
+            bad.append((r, int(p)))
+    assert not bad, f"{len(bad)} of {len(reqs)} requests differ from the reference (request, texture): {bad[:10]}"
+    if not serial:
+        assert stats["batches"] < len(reqs), "requests that arrive while others run are batched"

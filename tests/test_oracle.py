@@ -272,3 +272,24 @@ def test_colordist_error_zero_for_two_colour_blocks(oracle):
     for cd in (4, 5, 6, 0):
         blocks = oracle.encode_quantized(q, w, h, 0, cd, 0, 1, 0)
         assert oracle.colordist_error(q, w, h, 0, cd, blocks) == 0, cd
+
+
+def test_oracle_matches_serve_pool_digests(oracle):
+    """The serving pool's reference digests (one per texture: all mip levels,
+    70% quick mode, 30% nrandom 64 + LOOP) reproduced by the oracle for the
+    120 smallest textures, mip chains and DITHER_NONE quantization on the host."""
+    import hashlib
+    import json
+    from s2tc_amd import synth, workloads
+    exp = json.load(open(os.path.join(GOLDEN, "digests.json")))["workloads"]["serve_pool"]["texture_sha256"]
+    specs = synth.batch_specs(workloads.SERVE_POOL, workloads.SERVE_SEED, True)
+    order = sorted(range(len(specs)), key=lambda i: specs[i].width * specs[i].height)[:120]
+    assert any(specs[i].nrandom == 64 for i in order) and any(specs[i].nrandom < 0 for i in order)
+    for i in order:
+        s = specs[i]
+        blocks = []
+        for k, (w, h, lv) in enumerate(oracle.mip_chain(synth.texture(s), s.width, s.height)):
+            q = oracle.rgb565_image(lv, w, h, 4, {0: 1, 1: 4, 2: 8}[s.dxt], 0)
+            blocks.append(oracle.encode_quantized(q, w, h, s.dxt, s.cd, s.nrandom, s.refine,
+                                                  workloads.level_seed(s.seed, k)).tobytes())
+        assert hashlib.sha256(b"".join(blocks)).hexdigest() == exp[i], (i, s.width, s.height, s.cd, s.nrandom)
