@@ -39,12 +39,17 @@ extern "C" {
 #define S2TC_E_RANGE    (-4)   /* a size limit of the GPU path is exceeded */
 #define S2TC_E_BUDGET   (-5)   /* request exceeds the serving memory budget */
 
-/* largest nrandom the GPU path supports: the candidate list (16 + nrandom
- * colours, rounded up to 16, 4 bytes each), its distance rows (64 bytes
- * each) and the features of the first 16 candidates, for four blocks, must
- * fit the 64 KiB of LDS a workgroup can allocate: 224 candidates take
- * 61,696 bytes, 240 would take 66,048 */
-#define S2TC_MAX_NRANDOM 208
+/* Random candidates per block.  Up to S2TC_LDS_NRANDOM the candidate list
+ * (16 + nrandom colours, rounded up to 16, 4 bytes each), its distance rows
+ * (64 bytes each) and the features of the first 16 candidates of four
+ * blocks fit the 64 KiB of LDS a workgroup can allocate (224 candidates
+ * take 61,696 bytes, 240 would take 66,048).  Above it the same encoder
+ * keeps them in device scratch memory (m * 68 bytes per block in flight,
+ * allocated stream-ordered per call, or with a batch).  S2TC_MAX_NRANDOM
+ * bounds the candidate indices to 15 bits (16 + nrandom rounded up to 16
+ * stays at or below 32768); larger values are S2TC_E_RANGE. */
+#define S2TC_LDS_NRANDOM 208
+#define S2TC_MAX_NRANDOM 32752
 
 /* source kinds for s2tc_amd_encode_image */
 #define S2TC_SRC_QUANTIZED 0   /* rgb565_image output: r5 g6 b5 a, 4 bytes per texel */

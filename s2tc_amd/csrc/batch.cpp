@@ -33,6 +33,7 @@ struct s2tc_amd_batch {
 	std::vector<hipStream_t> cap_streams;
 	std::vector<hipEvent_t> cap_events;
 	void *d_desc = nullptr;  // device copy of every class's job and tile descriptors
+	void *d_scratch = nullptr;  // group memory of wide random-candidate classes
 };
 
 namespace s2tc {
@@ -80,6 +81,8 @@ static void free_plan(s2tc_amd_batch *b)
 	destroy_graph(b);
 	if (b->d_desc) (void)hipFree(b->d_desc);
 	b->d_desc = nullptr;
+	if (b->d_scratch) (void)hipFree(b->d_scratch);
+	b->d_scratch = nullptr;
 	for (auto s : b->cap_streams) (void)hipStreamDestroy(s);
 	for (auto e : b->cap_events) (void)hipEventDestroy(e);
 	b->cap_streams.clear();
@@ -107,6 +110,11 @@ int s2tc_amd_batch_create(const s2tc_amd_job *jobs, int njobs, s2tc_amd_batch **
 		if (!rc) {
 			plan_pack(b->classes, host.data(), (uint8_t *)b->d_desc);
 			rc = hip_check(hipMemcpy(b->d_desc, host.data(), bytes, hipMemcpyHostToDevice), "hipMemcpy (plan)");
+		}
+		const size_t sbytes = plan_scratch_bytes(b->classes);
+		if (!rc && sbytes) {
+			rc = hip_check(hipMalloc(&b->d_scratch, sbytes), "hipMalloc (batch scratch)");
+			if (!rc) plan_set_scratch(b->classes, (uint8_t *)b->d_scratch);
 		}
 	}
 	if (!rc) {

@@ -79,7 +79,7 @@ S2TC_D void row_min_key(uint32_t &hi, uint32_t &lo)
 }
 
 struct ScanKey {
-	uint32_t hi, lo;  // (sum, pair order i*1024+j); all-ones = none
+	uint32_t hi, lo;  // (sum, pair order i << 16 | j); all-ones = none
 	int qneg;         // largest pair order with a negative sum, -1 if none
 };
 
@@ -99,7 +99,7 @@ S2TC_D void scan_step(const int (&X)[16], const int (&Y)[16], int t, int p, int 
 	bool valid = i < m && j < m;
 	if (DIAG && R == 8) valid = valid && t < 8;
 	const int lo = i < j ? i : j, hi = i < j ? j : i;
-	const uint32_t order = (uint32_t)(lo * 1024 + hi);
+	const uint32_t order = ((uint32_t)lo << 16) | (uint32_t)hi;
 	if (PASS == 2) valid = valid && (int)order > qthr;
 	const bool better = valid && ((uint32_t)s < st.hi || ((uint32_t)s == st.hi && order < st.lo));
 	st.hi = better ? (uint32_t)s : st.hi;
@@ -163,7 +163,7 @@ S2TC_D void merge_key(uint32_t s, uint32_t order, bool ok, ScanKey &st)
 // The pair scan for rows of non-negative values whose 16-term sums do not
 // wrap (every distance but SRGB, and DXT5 alpha).  Within one row-block pair
 // a lane folds its pairs into one key per step: (sum << 4 | b) for the
-// partner b of block q > p (the pair order is i*1024 + 16q + b, so the
+// partner b of block q > p (the pair order is i << 16 | 16q + b, so the
 // smallest key is the smallest (sum, order)), (sum << 8 | lo << 4 | hi) for
 // pairs inside block p.  The partner's index travels through the DPP row
 // with its row values, all-ones when its row is outside the candidates, which
@@ -235,7 +235,7 @@ S2TC_D uint32_t reduce_rows_key(const int *rows, int t, int m, int pbw)
 			K kmin = ~(K)0;
 			KeyStepsDiag<1, 9, WIDE>::run(X, t, irow ? (uint32_t)t : 0xffffffffu, kmin);
 			const uint32_t lo = 16u * p + (uint32_t)((kmin >> 4) & 15u), hi = 16u * p + (uint32_t)(kmin & 15u);
-			merge_key((uint32_t)(kmin >> 8), lo * 1024u + hi, irow && kmin != ~(K)0, st);
+			merge_key((uint32_t)(kmin >> 8), (lo << 16) | hi, irow && kmin != ~(K)0, st);
 		}
 		for (int q = p + 1; q < pbw; ++q) {
 			int Y[16];
@@ -243,7 +243,7 @@ S2TC_D uint32_t reduce_rows_key(const int *rows, int t, int m, int pbw)
 			K kmin = ~(K)0;
 			KeyStepsOff<0, 16, WIDE>::run(X, Y, 16 * q + t < m ? (uint32_t)t : 0xffffffffu, kmin);
 			const bool ok = WIDE ? (kmin >> 32) < 8u : kmin != ~(K)0;
-			merge_key((uint32_t)(kmin >> 4), (uint32_t)i * 1024u + 16u * q + (uint32_t)(kmin & 15u), irow && ok, st);
+			merge_key((uint32_t)(kmin >> 4), ((uint32_t)i << 16) | (16u * q + (uint32_t)(kmin & 15u)), irow && ok, st);
 		}
 	}
 	row_min_key(st.hi, st.lo);
@@ -251,7 +251,8 @@ S2TC_D uint32_t reduce_rows_key(const int *rows, int t, int m, int pbw)
 }
 
 // reduce_colors_inplace / _2fixpoints argmin over the rows: returns the
-// winning pair order i*1024+j (group-uniform).
+// winning pair order i << 16 | j (group-uniform).  Candidate indices stay
+// below 2^15 (S2TC_MAX_NRANDOM), so every order is a positive int.
 template <bool NEG>
 S2TC_D uint32_t reduce_rows(const int *rows, int t, int m, int pbw)
 {
@@ -514,7 +515,7 @@ S2TC_D void encode_block_normal(uint32_t tex, int t, int w, int h, int dxt, int 
 	if constexpr (DistBound<CD>::value != 0) cbest = reduce_rows_key<false>(sm.rows, t, m, pbw);
 	else if constexpr (IsSymmetric<CD>::value) cbest = reduce_rows_key<true>(sm.rows, t, m, pbw);
 	else cbest = reduce_rows<true>(sm.rows, t, m, pbw);
-	const int ci = min((int)(cbest >> 10), m - 1), cj = min((int)(cbest & 1023u), m - 1);
+	const int ci = min((int)(cbest >> 16), m - 1), cj = min((int)(cbest & 0xffffu), m - 1);
 	uint32_t c0 = sm.cand[ci] & 0xffffu, c1 = sm.cand[cj] & 0xffffu;
 	int a0 = 0, a1 = 0;
 	// reduce_colors_inplace_2fixpoints for DXT5 alpha (:1006, :415-478).  The
@@ -543,7 +544,7 @@ S2TC_D void encode_block_normal(uint32_t tex, int t, int w, int h, int dxt, int 
 	__syncthreads();
 	if (dxt == DXT5) {
 		const uint32_t abest = reduce_rows_key<false>(sm.rows, t, m, pbw);
-		const int ai = min((int)(abest >> 10), m - 1), aj = min((int)(abest & 1023u), m - 1);
+		const int ai = min((int)(abest >> 16), m - 1), aj = min((int)(abest & 0xffffu), m - 1);
 		a0 = (int)(sm.cand[ai] >> 16);
 		a1 = (int)(sm.cand[aj] >> 16);
 	}

@@ -22,10 +22,12 @@ struct KernelTable {
 	const void *fast[3][2];       // [dxt][raw]: k_fast<CD, dxt, raw>, null for NORMALMAP
 	const void *fast_int[3][2][3];  // [dxt][raw][refine]: k_fast_interior (full blocks, 16-byte rows), null for NORMALMAP
 	const void *normal[2];        // [raw]: k_normal<CD, raw>
+	const void *normal_wide[2];   // [raw]: k_normal_wide<CD, raw> (group memory in global scratch)
 	int normal_smem_per_cand;     // LDS bytes per candidate slot for a 4-block workgroup
 	int normal_smem_fixed;        // plus these bytes (features of the 16 texel candidates of 4 blocks)
 	const void *fast_batch[3][3]; // [dxt][refine]: persistent quick-mode kernel over a job list
 	const void *normal_batch;     // persistent MODE_NORMAL kernel over a job list
+	const void *normal_batch_wide;  // the same with group memory in global scratch
 	const void *all16[3][2];      // [dxt][raw]: MODE_NORMAL without random colours, one lane per block
 	const void *all16_batch[3][3];  // [dxt][refine]: its persistent batch form
 	const void *all16_batch_alone[3][3];  // the form for a batch of this one class

@@ -166,17 +166,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(All16Waves<
 	store_block<DXT>(dst, b, o);
 }
 
-// MODE_NORMAL over one image: 16 lanes per block, 4 blocks per 64-lane workgroup.
+// MODE_NORMAL over one image: 16 lanes per block, 4 blocks per 64-lane
+// workgroup; tile `tile` covers raster blocks 4*tile .. 4*tile+3.
 template <int CD, bool RAW>
-__global__ __launch_bounds__(64) void k_normal(const uint32_t *__restrict__ src, int width, int height,
-                                               uint8_t *__restrict__ dst, int dxt, int nrandom, int refine,
-                                               unsigned long long seed, int pbw, unsigned block_base)
+S2TC_D void normal_image_tile(const uint32_t *__restrict__ src, int width, int height, uint8_t *__restrict__ dst,
+                              int dxt, int nrandom, int refine, unsigned long long seed, int pbw,
+                              unsigned block_base, const GroupSmem<CD> &sm, int tile, int g, int t)
 {
-	extern __shared__ __align__(16) uint8_t smem[];
-	const int lane = (int)threadIdx.x;
-	const int g = lane >> 4, t = lane & 15;
 	const int bw = (width + 3) >> 2, bh = (height + 3) >> 2, nb = bw * bh;
-	int b = (int)blockIdx.x * 4 + g;
+	int b = tile * 4 + g;
 	const bool have = b < nb;
 	if (!have) b = nb - 1;
 	const int by = b / bw, bx = b - by * bw;
@@ -188,12 +186,50 @@ __global__ __launch_bounds__(64) void k_normal(const uint32_t *__restrict__ src,
 		if (RAW) tex = quant_none_rt(tex, dxt);
 	}
 	const uint32_t bkey = s2tc_prng_block_key((uint64_t)seed, block_base + (uint32_t)b);
-	const GroupSmem<S2TC_CD> sm = group_smem<S2TC_CD>(smem, g, 16 * pbw);
 	BlockOut o;
-	encode_block_normal<S2TC_CD>(tex, t, w, h, dxt, nrandom, refine, bkey, pbw, sm, o);
+	encode_block_normal<CD>(tex, t, w, h, dxt, nrandom, refine, bkey, pbw, sm, o);
 	if (have && t == 0) {
 		if (dxt == DXT1) store_block<DXT1>(dst, (size_t)b, o);
 		else store_block<DXT5>(dst, (size_t)b, o);
+	}
+}
+
+// Candidates, rows and column features in LDS (nrandom <= S2TC_LDS_NRANDOM);
+// one workgroup per tile.
+template <int CD, bool RAW>
+__global__ __launch_bounds__(64) void k_normal(const uint32_t *__restrict__ src, int width, int height,
+                                               uint8_t *__restrict__ dst, int dxt, int nrandom, int refine,
+                                               unsigned long long seed, int pbw, unsigned block_base)
+{
+	extern __shared__ __align__(16) uint8_t smem[];
+	const int lane = (int)threadIdx.x;
+	const int g = lane >> 4, t = lane & 15;
+	const GroupSmem<CD> sm = group_smem<CD>(smem, g, 16 * pbw);
+	normal_image_tile<CD, RAW>(src, width, height, dst, dxt, nrandom, refine, seed, pbw, block_base, sm,
+	                           (int)blockIdx.x, g, t);
+}
+
+// More candidates than LDS holds: each workgroup keeps its four blocks'
+// candidates, rows and column features in its own slice of `gmem`
+// (4 * group_smem_bytes(16 * pbw) bytes per workgroup, gridDim.x slices) and
+// strides over the tiles.  The slice is reused tile after tile, so a
+// barrier ends every tile.
+template <int CD, bool RAW>
+__global__ __launch_bounds__(64) void k_normal_wide(const uint32_t *__restrict__ src, int width, int height,
+                                                    uint8_t *__restrict__ dst, int dxt, int nrandom, int refine,
+                                                    unsigned long long seed, int pbw, unsigned block_base,
+                                                    uint8_t *__restrict__ gmem)
+{
+	const int lane = (int)threadIdx.x;
+	const int g = lane >> 4, t = lane & 15;
+	const size_t wg_bytes = 4 * group_smem_bytes<CD>(16 * pbw);
+	const GroupSmem<CD> sm = group_smem<CD>(gmem + (size_t)blockIdx.x * wg_bytes, g, 16 * pbw);
+	const int bw = (width + 3) >> 2, bh = (height + 3) >> 2;
+	const int ntiles = (bw * bh + 3) >> 2;
+	for (int tile = (int)blockIdx.x; tile < ntiles; tile += (int)gridDim.x) {
+		normal_image_tile<CD, RAW>(src, width, height, dst, dxt, nrandom, refine, seed, pbw, block_base, sm, tile,
+		                           g, t);
+		__syncthreads();
 	}
 }
 
@@ -291,6 +327,35 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 }
 
 template <int CD>
+S2TC_D void normal_batch_tile(const JobDesc *__restrict__ jobs, const TileJobs *__restrict__ tile_jobs,
+                              long long nblocks, const GroupSmem<CD> &sm, int tile, int g, int t)
+{
+	long long gb = (long long)tile * 4 + g;
+	const bool have = gb < nblocks;
+	if (!have) gb = nblocks - 1;
+	const TileJobs tj = tile_jobs[tile];
+	const JobDesc &j = jobs[find_job(jobs, tj.lo, tj.hi, gb)];
+	const int b = (int)(gb - j.block0);
+	const int bw = (j.width + 3) >> 2;
+	const int by = b / bw, bx = b - by * bw;
+	const int w = min(4, j.width - 4 * bx), h = min(4, j.height - 4 * by);
+	const int x = t >> 2, y = t & 3;
+	uint32_t tex = 0;
+	if (x < w && y < h) {
+		tex = j.src[(size_t)(4 * by + y) * (size_t)j.width + 4 * bx + x];
+		if (j.raw) tex = quant_none_rt(tex, j.dxt);
+	}
+	const uint32_t bkey = s2tc_prng_block_key((uint64_t)j.seed, (uint32_t)b);
+	const int pbw = (16 + (j.nrandom > 0 ? j.nrandom : 0) + 15) / 16;
+	BlockOut o;
+	encode_block_normal<CD>(tex, t, w, h, j.dxt, j.nrandom, j.refine, bkey, pbw, sm, o);
+	if (have && t == 0) {
+		if (j.dxt == DXT1) store_block<DXT1>(j.dst, (size_t)b, o);
+		else store_block<DXT5>(j.dst, (size_t)b, o);
+	}
+}
+
+template <int CD>
 __global__ __launch_bounds__(64) void k_normal_batch(const JobDesc *__restrict__ jobs, const TileJobs *__restrict__ tile_jobs,
                                                      long long nblocks, int ntiles, int pbw_max)
 {
@@ -299,29 +364,24 @@ __global__ __launch_bounds__(64) void k_normal_batch(const JobDesc *__restrict__
 	const int g = lane >> 4, t = lane & 15;
 	const GroupSmem<CD> sm = group_smem<CD>(smem, g, 16 * pbw_max);
 	for (int tile = (int)blockIdx.x; tile < ntiles; tile += (int)gridDim.x) {
-		long long gb = (long long)tile * 4 + g;
-		const bool have = gb < nblocks;
-		if (!have) gb = nblocks - 1;
-		const TileJobs tj = tile_jobs[tile];
-		const JobDesc &j = jobs[find_job(jobs, tj.lo, tj.hi, gb)];
-		const int b = (int)(gb - j.block0);
-		const int bw = (j.width + 3) >> 2;
-		const int by = b / bw, bx = b - by * bw;
-		const int w = min(4, j.width - 4 * bx), h = min(4, j.height - 4 * by);
-		const int x = t >> 2, y = t & 3;
-		uint32_t tex = 0;
-		if (x < w && y < h) {
-			tex = j.src[(size_t)(4 * by + y) * (size_t)j.width + 4 * bx + x];
-			if (j.raw) tex = quant_none_rt(tex, j.dxt);
-		}
-		const uint32_t bkey = s2tc_prng_block_key((uint64_t)j.seed, (uint32_t)b);
-		const int pbw = (16 + (j.nrandom > 0 ? j.nrandom : 0) + 15) / 16;
-		BlockOut o;
-		encode_block_normal<CD>(tex, t, w, h, j.dxt, j.nrandom, j.refine, bkey, pbw, sm, o);
-		if (have && t == 0) {
-			if (j.dxt == DXT1) store_block<DXT1>(j.dst, (size_t)b, o);
-			else store_block<DXT5>(j.dst, (size_t)b, o);
-		}
+		normal_batch_tile<CD>(jobs, tile_jobs, nblocks, sm, tile, g, t);
+		__syncthreads();
+	}
+}
+
+// The batch form of k_normal_wide: group memory in a global slice per
+// workgroup (4 * group_smem_bytes(16 * pbw_max) bytes each).
+template <int CD>
+__global__ __launch_bounds__(64) void k_normal_batch_wide(const JobDesc *__restrict__ jobs,
+                                                          const TileJobs *__restrict__ tile_jobs, long long nblocks,
+                                                          int ntiles, int pbw_max, uint8_t *__restrict__ gmem)
+{
+	const int lane = (int)threadIdx.x;
+	const int g = lane >> 4, t = lane & 15;
+	const size_t wg_bytes = 4 * group_smem_bytes<CD>(16 * pbw_max);
+	const GroupSmem<CD> sm = group_smem<CD>(gmem + (size_t)blockIdx.x * wg_bytes, g, 16 * pbw_max);
+	for (int tile = (int)blockIdx.x; tile < ntiles; tile += (int)gridDim.x) {
+		normal_batch_tile<CD>(jobs, tile_jobs, nblocks, sm, tile, g, t);
 		__syncthreads();
 	}
 }
@@ -423,6 +483,7 @@ extern "C" void S2TC_CAT(s2tc_kernels_cd, S2TC_CD)(KernelTable *kt)
 	for (int d = 0; d < 3; ++d) kt->fast_batch[d][0] = kt->fast_batch[d][1] = kt->fast_batch[d][2] = nullptr;
 #endif
 	kt->normal_batch = (const void *)k_normal_batch<S2TC_CD>;
+	kt->normal_batch_wide = (const void *)k_normal_batch_wide<S2TC_CD>;
 	kt->error[DXT1][0] = (const void *)k_colordist_error<S2TC_CD, DXT1, false>;
 	kt->error[DXT3][0] = (const void *)k_colordist_error<S2TC_CD, DXT3, false>;
 	kt->error[DXT5][0] = (const void *)k_colordist_error<S2TC_CD, DXT5, false>;
@@ -453,6 +514,8 @@ extern "C" void S2TC_CAT(s2tc_kernels_cd, S2TC_CD)(KernelTable *kt)
 #undef S2TC_AB3
 	kt->normal[0] = (const void *)k_normal<S2TC_CD, false>;
 	kt->normal[1] = (const void *)k_normal<S2TC_CD, true>;
+	kt->normal_wide[0] = (const void *)k_normal_wide<S2TC_CD, false>;
+	kt->normal_wide[1] = (const void *)k_normal_wide<S2TC_CD, true>;
 #if defined(S2TC_NM_TABLE) || defined(S2TC_SRGB_Y_TABLE)
 	kt->init = (const void *)k_tables_init<S2TC_CD>;
 #else

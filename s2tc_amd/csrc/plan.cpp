@@ -25,8 +25,9 @@ int plan_classes(const s2tc_amd_job *jobs, int njobs, std::vector<ClassPlan> &cl
 	classes.clear();
 	// class slots: 0..71 quick ((cd*3 + dxt)*3 + refine), 72..79 MODE_NORMAL
 	// with random colours (cd), 80..151 MODE_NORMAL without
-	// (80 + (cd*3 + dxt)*3 + refine)
-	int slot[152];
+	// (80 + (cd*3 + dxt)*3 + refine), 152..159 MODE_NORMAL with more random
+	// colours than LDS holds (152 + cd)
+	int slot[160];
 	for (int &v : slot) v = -1;
 	for (int i = 0; i < njobs; ++i) {
 		const s2tc_amd_job &j = jobs[i];
@@ -40,12 +41,16 @@ int plan_classes(const s2tc_amd_job *jobs, int njobs, std::vector<ClassPlan> &cl
 			return set_error(S2TC_E_ARG, "job %d: bad source kind", i);
 		const bool fast = is_fast(cd, j.nrandom);
 		const bool all16 = !fast && j.nrandom <= 0;
-		const int key = fast ? (cd * 3 + dxt) * 3 + refine : all16 ? 80 + (cd * 3 + dxt) * 3 + refine : 72 + cd;
+		const bool wide = !fast && j.nrandom > S2TC_LDS_NRANDOM;
+		const int key = fast ? (cd * 3 + dxt) * 3 + refine
+		                : all16 ? 80 + (cd * 3 + dxt) * 3 + refine
+		                : wide ? 152 + cd : 72 + cd;
 		if (slot[key] < 0) {
 			slot[key] = (int)classes.size();
 			classes.emplace_back();
 			ClassPlan &c = classes.back();
 			c.normal = !fast && !all16;
+			c.wide = wide;
 			c.all16 = all16;
 			c.cd = cd;
 			c.dxt = c.normal ? -1 : dxt;
@@ -78,10 +83,14 @@ int plan_classes(const s2tc_amd_job *jobs, int njobs, std::vector<ClassPlan> &cl
 	}
 	for (auto &c : classes) {
 		const KernelTable &kt = kernel_table(c.cd);
-		c.kernel = c.normal ? kt.normal_batch : c.all16 ? kt.all16_batch[c.dxt][c.refine] : kt.fast_batch[c.dxt][c.refine];
+		c.kernel = c.wide     ? kt.normal_batch_wide
+		           : c.normal ? kt.normal_batch
+		           : c.all16  ? kt.all16_batch[c.dxt][c.refine]
+		                      : kt.fast_batch[c.dxt][c.refine];
 		if (!c.kernel) return set_error(S2TC_E_ARG, "no batch kernel for distance %d", c.cd);
 		c.ntiles = (int)((c.nblocks + c.tile - 1) / c.tile);
-		c.smem = c.normal ? (size_t)kt.normal_smem_per_cand * 16 * c.pbw_max + (size_t)kt.normal_smem_fixed : 0;
+		c.smem = c.normal && !c.wide ? (size_t)kt.normal_smem_per_cand * 16 * c.pbw_max + (size_t)kt.normal_smem_fixed : 0;
+		if (c.wide) wide_launch_shape(kt, c.pbw_max, c.ntiles, device_cu_count(), c.wide_grid, c.scratch);
 		// tile -> (first job, last job), by a walk over the packed blocks
 		c.tiles.resize((size_t)c.ntiles);
 		int jlo = 0;
@@ -126,6 +135,32 @@ void plan_pack(std::vector<ClassPlan> &classes, uint8_t *host, uint8_t *dev)
 	}
 }
 
+static size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+size_t plan_scratch_bytes(const std::vector<ClassPlan> &classes)
+{
+	size_t n = 0;
+	for (const auto &c : classes) n += align256(c.scratch);
+	return n;
+}
+
+void plan_set_scratch(std::vector<ClassPlan> &classes, uint8_t *dev)
+{
+	size_t off = 0;
+	for (auto &c : classes) {
+		c.d_scratch = c.scratch ? dev + off : nullptr;
+		off += align256(c.scratch);
+	}
+}
+
+void wide_launch_shape(const KernelTable &kt, int pbw, long long ntiles, int ncu, int &grid, size_t &bytes)
+{
+	const size_t wg = (size_t)kt.normal_smem_per_cand * 16 * (size_t)pbw + (size_t)kt.normal_smem_fixed;
+	const long long cap = std::max(1LL, (long long)(((size_t)1 << 30) / wg));
+	grid = (int)std::max(1LL, std::min(ntiles, std::min(4LL * ncu, cap)));
+	bytes = wg * (size_t)grid;
+}
+
 int plan_launch_class(const ClassPlan &c, int wg_per_cu, int ncu, hipStream_t st)
 {
 	long long nb = c.nblocks;
@@ -134,6 +169,14 @@ int plan_launch_class(const ClassPlan &c, int wg_per_cu, int ncu, hipStream_t st
 	if (grid < 1) return 0;
 	const JobDesc *jobs = c.d_jobs;
 	const TileJobs *tiles = c.d_tiles;
+	if (c.wide) {
+		int pbw = c.pbw_max;
+		uint8_t *gmem = c.d_scratch;
+		if (!gmem) return set_error(S2TC_E_ARG, "wide class without scratch memory");
+		void *args[] = {(void *)&jobs, (void *)&tiles, (void *)&nb, (void *)&ntiles, (void *)&pbw, (void *)&gmem};
+		return hip_check(hipLaunchKernel(c.kernel, dim3(c.wide_grid), dim3(64), args, 0, st),
+		                 "launch k_normal_batch_wide");
+	}
 	if (c.normal) {
 		int pbw = c.pbw_max;
 		void *args[] = {(void *)&jobs, (void *)&tiles, (void *)&nb, (void *)&ntiles, (void *)&pbw};

@@ -14,9 +14,12 @@ namespace s2tc {
 //   MODE_NORMAL without random colours: one class per (distance, format,
 //     refinement), k_all16_batch, one lane per block, tiles of 256;
 //   MODE_NORMAL with random colours: one class per distance (formats mixed),
-//     k_normal_batch, 16 lanes per block, tiles of 4 blocks.
+//     k_normal_batch, 16 lanes per block, tiles of 4 blocks; jobs with more
+//     than S2TC_LDS_NRANDOM random colours form their own class per distance
+//     (k_normal_batch_wide, group memory in device scratch).
 struct ClassPlan {
 	bool normal = false;         // 16-lanes-per-block kernel
+	bool wide = false;           // ... with its group memory in device scratch
 	bool all16 = false;          // one-lane-per-block MODE_NORMAL kernel
 	int cd = 0, dxt = -1;
 	int refine = -1;             // one-lane classes: the refinement all their jobs use
@@ -26,6 +29,9 @@ struct ClassPlan {
 	long long nblocks = 0, texels = 0, bytes = 0;
 	int ntiles = 0, tile = 256, pbw_max = 1;
 	size_t smem = 0;
+	size_t scratch = 0;          // wide classes: device scratch bytes (wide_grid slices)
+	int wide_grid = 0;           // wide classes: workgroups of every launch
+	uint8_t *d_scratch = nullptr;
 	const void *kernel = nullptr;
 	JobDesc *d_jobs = nullptr;   // device addresses (set by plan_pack)
 	TileJobs *d_tiles = nullptr;
@@ -41,6 +47,14 @@ size_t plan_descriptor_bytes(const std::vector<ClassPlan> &classes);
 // Lays the descriptors out for a device buffer at `dev`: writes them to the
 // host staging buffer `host` (same layout) and sets d_jobs / d_tiles.
 void plan_pack(std::vector<ClassPlan> &classes, uint8_t *host, uint8_t *dev);
+// Bytes of device scratch memory the classes need (wide classes only), and
+// its assignment to the classes from a buffer at `dev`.
+size_t plan_scratch_bytes(const std::vector<ClassPlan> &classes);
+void plan_set_scratch(std::vector<ClassPlan> &classes, uint8_t *dev);
+// Workgroups and scratch bytes of a wide random-candidate launch over
+// `ntiles` tiles with `pbw` row blocks: slices of 4 * group memory, at most
+// 4 workgroups per CU and about 1 GiB in total.
+void wide_launch_shape(const KernelTable &kt, int pbw, long long ntiles, int ncu, int &grid, size_t &bytes);
 // One launch of a class kernel: a grid of wg_per_cu * ncu workgroups (all
 // tiles when wg_per_cu is 0) striding over the class's tiles.
 int plan_launch_class(const ClassPlan &c, int wg_per_cu, int ncu, hipStream_t st);

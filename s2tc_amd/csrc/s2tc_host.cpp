@@ -7,6 +7,7 @@
 // NORMALMAP; MODE_NORMAL without random colours (nrandom <= 0) runs the
 // one-lane-per-block all-16 kernel, MODE_NORMAL with random colours the
 // 16-lanes-per-block kernel.
+#include "plan.h"
 #include "s2tc_internal.h"
 
 #include <cstdarg>
@@ -156,11 +157,27 @@ int launch_encode(const EncodeArgs &a, hipStream_t stream)
 	int pbw = normal_pbw(nrandom);
 	unsigned long long seed = a.seed;
 	unsigned block_base = a.block_index_base;
-	size_t smem = (size_t)kt.normal_smem_per_cand * 16 * pbw + (size_t)kt.normal_smem_fixed;
-	void *args[] = {(void *)&src, (void *)&width, (void *)&height, (void *)&dst, (void *)&dxt,
-	                (void *)&nrandom, (void *)&refine, (void *)&seed, (void *)&pbw, (void *)&block_base};
-	dim3 grid((unsigned)((nb + 3) / 4)), block(64);
-	return hip_check(hipLaunchKernel(kt.normal[raw], grid, block, args, smem, stream), "launch k_normal");
+	if (nrandom <= S2TC_LDS_NRANDOM) {
+		size_t smem = (size_t)kt.normal_smem_per_cand * 16 * pbw + (size_t)kt.normal_smem_fixed;
+		void *args[] = {(void *)&src, (void *)&width, (void *)&height, (void *)&dst, (void *)&dxt,
+		                (void *)&nrandom, (void *)&refine, (void *)&seed, (void *)&pbw, (void *)&block_base};
+		dim3 grid((unsigned)((nb + 3) / 4)), block(64);
+		return hip_check(hipLaunchKernel(kt.normal[raw], grid, block, args, smem, stream), "launch k_normal");
+	}
+	// more candidates than LDS holds: group memory in stream-ordered scratch
+	// that lives exactly as long as this launch
+	int grid = 0;
+	size_t bytes = 0;
+	wide_launch_shape(kt, pbw, (nb + 3) / 4, device_cu_count(), grid, bytes);
+	void *gmem = nullptr;
+	if (hipMallocAsync(&gmem, bytes, stream) != hipSuccess)
+		return set_error(S2TC_E_RANGE, "cannot allocate %zu bytes of scratch for nrandom %d", bytes, nrandom);
+	void *args[] = {(void *)&src, (void *)&width, (void *)&height, (void *)&dst, (void *)&dxt, (void *)&nrandom,
+	                (void *)&refine, (void *)&seed, (void *)&pbw, (void *)&block_base, (void *)&gmem};
+	const int rc = hip_check(hipLaunchKernel(kt.normal_wide[raw], dim3((unsigned)grid), dim3(64), args, 0, stream),
+	                         "launch k_normal_wide");
+	const int rc2 = hip_check(hipFreeAsync(gmem, stream), "hipFreeAsync (scratch)");
+	return rc ? rc : rc2;
 }
 
 }  // namespace s2tc

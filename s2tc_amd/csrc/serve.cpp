@@ -73,6 +73,8 @@ struct Slot {
 	hipEvent_t ev = nullptr;
 	void *d_desc = nullptr;
 	size_t d_cap = 0;
+	void *d_scratch = nullptr;  // group memory of wide random-candidate classes
+	size_t s_cap = 0;
 	uint8_t *h_desc = nullptr;
 	size_t h_cap = 0;
 	std::vector<int> reqs;
@@ -93,6 +95,17 @@ static int grow_slot(Slot &s, size_t bytes)
 	if (int r = hip_check(hipMalloc(&s.d_desc, bytes), "hipMalloc (serve descriptors)")) return r;
 	if (int r = hip_check(hipHostMalloc((void **)&s.h_desc, bytes, hipHostMallocDefault), "hipHostMalloc")) return r;
 	s.d_cap = s.h_cap = bytes;
+	return 0;
+}
+
+static int grow_scratch(Slot &s, size_t bytes)
+{
+	if (bytes <= s.s_cap) return 0;
+	if (s.d_scratch) (void)hipFree(s.d_scratch);
+	s.d_scratch = nullptr;
+	s.s_cap = 0;
+	if (int r = hip_check(hipMalloc(&s.d_scratch, bytes), "hipMalloc (serve scratch)")) return r;
+	s.s_cap = bytes;
 	return 0;
 }
 
@@ -231,7 +244,10 @@ extern "C" int s2tc_amd_serve(const s2tc_amd_job *jobs, int njobs, const s2tc_am
 				rc = plan_classes(bjobs.data(), (int)bjobs.size(), classes);
 				const size_t db = rc ? 0 : plan_descriptor_bytes(classes);
 				if (!rc) rc = grow_slot(s, db);
+				const size_t sb = rc ? 0 : plan_scratch_bytes(classes);
+				if (!rc && sb) rc = grow_scratch(s, sb);
 				if (!rc) {
+					plan_set_scratch(classes, (uint8_t *)s.d_scratch);
 					plan_pack(classes, s.h_desc, (uint8_t *)s.d_desc);
 					rc = hip_check(hipMemcpyAsync(s.d_desc, s.h_desc, db, hipMemcpyHostToDevice, s.st), "H2D (descriptors)");
 				}
@@ -269,6 +285,7 @@ extern "C" int s2tc_amd_serve(const s2tc_amd_job *jobs, int njobs, const s2tc_am
 		if (s.ev) (void)hipEventDestroy(s.ev);
 		if (s.st) (void)hipStreamDestroy(s.st);
 		if (s.d_desc) (void)hipFree(s.d_desc);
+		if (s.d_scratch) (void)hipFree(s.d_scratch);
 		if (s.h_desc) (void)hipHostFree(s.h_desc);
 	}
 	(void)hipFree(arena_mem);

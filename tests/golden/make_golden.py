@@ -12,6 +12,8 @@ Files written:
                 (dxt, distance, nrandom, refine)
   dither.npz    rgb565_image outputs for every dither/srccomps/alphabits
   compress.npz  tx_compress_dxtn outputs (environment settings, row strides)
+  wide.npz      tx_compress_dxtn outputs with S2TC_RANDOM_COLORS above the
+                GPU's LDS-resident limit (209 .. 3000 random colours)
   digests.json  SHA-256 of full-size workload outputs and of the reference
                 CLI's DDS files for the test images
   from_s3tc.npz DDS files of random S3TC blocks and the reference
@@ -124,6 +126,42 @@ def compress_corpus(ref):
     return len(cases)
 
 
+def wide_corpus(ref):
+    """tx_compress_dxtn through the reference with many random colours: the
+    reference sizes its candidate and distance arrays by 16 + nrandom
+    (s2tc_algorithm.cpp:875-876), so any positive value is legal."""
+    rng = np.random.default_rng(29)
+    out = {}
+    cases = []
+    for (w, h) in [(13, 9), (8, 8)]:
+        src = rng.integers(0, 256, w * h * 4).astype(np.uint8)
+        src[3::4][rng.random(w * h) < 0.3] = 0  # some transparent texels for DXT1
+        out[f"src_{w}_{h}"] = src
+    settings = [(7, 209, 2), (7, 300, 1), (2, 256, 2), (0, 300, 2), (3, 1000, 0), (5, 3000, 1), (1, 433, 2)]
+    for (w, h) in [(13, 9), (8, 8)]:
+        src = out[f"src_{w}_{h}"]
+        bw, bh = (w + 3) // 4, (h + 3) // 4
+        for fmt in (0x83F1, 0x83F2, 0x83F3):
+            bs = 8 if fmt < 0x83F2 else 16
+            for (cd, nr, refine) in settings:
+                if nr == 3000 and (w, h) != (8, 8):
+                    continue
+                env = dict(S2TC_DITHER_MODE="NONE", S2TC_COLORDIST_MODE=NAMES[cd], S2TC_RANDOM_COLORS=str(nr),
+                           S2TC_REFINE_COLORS=REFINES[refine], S2TC_RANDOM_SEED="5")
+                os.environ.update(env)
+                dst = np.full(bw * bh * bs, 0xAB, np.uint8)
+                ref.lib.tx_compress_dxtn(4, w, h, src.ctypes.data, fmt, dst.ctypes.data, bw * bs)
+                key = f"{w}_{h}_{fmt}_{cd}_{nr}_{refine}"
+                out["dst_" + key] = dst
+                cases.append(key)
+    for k in ("S2TC_DITHER_MODE", "S2TC_COLORDIST_MODE", "S2TC_RANDOM_COLORS", "S2TC_REFINE_COLORS",
+              "S2TC_RANDOM_SEED"):
+        os.environ.pop(k, None)
+    out["cases"] = np.array(cases)
+    np.savez_compressed(os.path.join(HERE, "wide.npz"), **out)
+    return len(cases)
+
+
 def cli_digests():
     """DDS files of the reference CLI (oracle/_ref/s2tc_compress with the
     reference library) for the reference's test images and settings."""
@@ -140,6 +178,25 @@ def cli_digests():
                     subprocess.run([cli, "-i", os.path.join(tga_dir, name), "-o", o, "-t", fmt, "-l", lib],
                                    check=True, env=e, capture_output=True)
                     res[f"{name}/{fmt}/{setting}"] = hashlib.sha256(open(o, "rb").read()).hexdigest()
+    return res
+
+
+def cli_workload_digests():
+    """The reference CLI (default settings: SIMPLE dither, WAVG, quick mode,
+    REFINE_ALWAYS, whole mip chain in the DDS) on the 256x256 sinusoid
+    workload written as a 32-bit TGA, DXT1."""
+    cli = os.path.join(ROOT, "oracle", "_ref", "s2tc_compress")
+    lib = os.path.join(ROOT, "oracle", "_ref", "libs2tc_ref.so")
+    wl = workloads.IMAGE_WORKLOADS["sinus256"]
+    src = workloads.make_image_source(wl)
+    res = {}
+    env = {k: v for k, v in os.environ.items() if not k.startswith("S2TC_")}
+    with tempfile.TemporaryDirectory() as td:
+        t, o = os.path.join(td, "i.tga"), os.path.join(td, "o.dds")
+        imgutil.write_tga32(t, src, wl["width"], wl["height"])
+        subprocess.run([cli, "-i", t, "-o", o, "-t", "DXT1", "-l", lib], check=True, env=env, capture_output=True)
+        res["sinus256"] = {"tga_sha256": hashlib.sha256(open(t, "rb").read()).hexdigest(),
+                           "dds_sha256": hashlib.sha256(open(o, "rb").read()).hexdigest()}
     return res
 
 
@@ -283,8 +340,12 @@ def main():
         print("dither cases", dither_corpus(ref))
     if not only or "compress" in only:
         print("compress cases", compress_corpus(ref))
+    if not only or "wide" in only:
+        print("wide cases", wide_corpus(ref))
     if not only or "cli" in only:
         digests["cli"] = cli_digests()
+    if not only or "cli_workloads" in only:
+        digests["cli_workloads"] = cli_workload_digests()
     if not only or "decompress" in only:
         digests["decompress"] = decompress_digests()
     if not only or "from_s3tc" in only:

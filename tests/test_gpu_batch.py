@@ -64,6 +64,42 @@ def test_batch_equals_oracle_and_plain(gpu, oracle):
     imgs.free()
 
 
+def test_batch_wide_class(gpu, oracle):
+    """Jobs with more random colours than LDS holds form their own class per
+    distance (group memory in scratch that the batch allocates), next to
+    LDS-resident random-colour jobs of the same distance and quick jobs:
+    every schedule equals the oracle."""
+    rng = np.random.default_rng(31)
+    items = []
+    for (w, h, dxt, cd, nr, rf) in [(9, 7, 0, 7, 300, 2), (16, 8, 2, 7, 64, 2), (5, 5, 1, 7, 211, 1),
+                                    (8, 8, 2, 2, 250, 2), (12, 4, 0, 2, 16, 1), (13, 9, 2, 5, -1, 1),
+                                    (3, 2, 2, 0, 500, 0)]:
+        items.append(dict(rgba=rng.integers(0, 256, w * h * 4).astype(np.uint8), width=w, height=h, dxt=dxt,
+                          cd=cd, nrandom=nr, refine=rf, seed=int(rng.integers(0, 2**63))))
+    imgs = gpu.DeviceImages(items)
+    b = gpu.Batch(imgs)
+    for sched in [(4, 4), (0, 1), (1, 8)]:
+        imgs.dst.fill(0)
+        b.set_schedule(*sched)
+        _say(f"batch with wide classes, schedule {sched}")
+        b.launch()
+        gpu.synchronize()
+        host = imgs.dst.to_host()
+        for i, it in enumerate(items):
+            q = oracle.rgb565_image(it["rgba"], it["width"], it["height"], 4, {0: 1, 1: 4, 2: 8}[it["dxt"]], 0)
+            exp = oracle.encode_quantized(q, it["width"], it["height"], it["dxt"], it["cd"], it["nrandom"],
+                                          it["refine"], it["seed"])
+            assert np.array_equal(imgs.output(i, host), exp), (sched, i, {k: v for k, v in it.items() if k != "rgba"})
+    _say("serve the same images as requests")
+    outs = [np.zeros_like(imgs.output(i, host)) for i in range(len(items))]
+    stats, lat = gpu.serve(imgs, [(1e-5 * i, i, 1) for i in range(len(items))], streams=2, keep_outputs=outs)
+    assert stats["completed"] == len(items)
+    for i in range(len(items)):
+        assert np.array_equal(outs[i], imgs.output(i, host)), i
+    b.destroy()
+    imgs.free()
+
+
 def test_device_mip_sources_match_host_chain(gpu, oracle):
     """DeviceImages.from_textures: every level built on the device equals the
     oracle's Image_MipReduce32 chain of the base level, seeds follow

@@ -184,10 +184,32 @@ def test_concurrent_tx_compress_callers(gpu, clean_env):
 
 
 def test_nrandom_limit_is_an_error(gpu):
+    """Candidate indices are 15-bit: nrandom above 32752 is refused with an
+    error (the output is not written); 32752 itself is accepted."""
     src = gpu.DeviceBuffer(64)
     dst = gpu.DeviceBuffer(16)
     with pytest.raises(gpu.S2tcError, match="nrandom"):
-        gpu.encode_image_device(src, 0, 4, 4, dst, 0, 5, 209, 1)
+        gpu.encode_image_device(src, 0, 4, 4, dst, 0, 5, 32753, 1)
+
+
+def test_tx_compress_dxtn_wide_matches_golden(gpu, clean_env):
+    """S2TC_RANDOM_COLORS from 209 to 3000 (candidate lists beyond what LDS
+    holds: the encoder keeps them in device scratch) through
+    tx_compress_dxtn: the reference's bytes for every format."""
+    L = gpu.lib()
+    L.tx_compress_dxtn.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint,
+                                   ctypes.c_void_p, ctypes.c_int]
+    z = load("wide.npz")
+    for key in z["cases"]:
+        w, h, fmt, cd, nr, refine = map(int, str(key).split("_"))
+        os.environ.update(S2TC_DITHER_MODE="NONE", S2TC_COLORDIST_MODE=NAMES[cd], S2TC_RANDOM_COLORS=str(nr),
+                          S2TC_REFINE_COLORS=REFINES[refine], S2TC_RANDOM_SEED="5")
+        _say(f"tx_compress_dxtn wide {key}")
+        exp = z["dst_" + str(key)]
+        dst = np.full(exp.size, 0xAB, np.uint8)
+        bs = 8 if fmt < 0x83F2 else 16
+        L.tx_compress_dxtn(4, w, h, z[f"src_{w}_{h}"].ctypes.data, fmt, dst.ctypes.data, ((w + 3) // 4) * bs)
+        assert np.array_equal(dst, exp), key
 
 
 @pytest.mark.parametrize("w,h,sc", [(2049, 2049, 3), (2049, 1025, 4), (1531, 1373, 4)])

@@ -139,7 +139,7 @@ def test_uniform_alpha_blocks(gpu, oracle, raw):
 def test_random_candidate_counts(gpu, oracle, cd):
     """Random-candidate blocks with candidate counts 16 + nrandom that are not
     multiples of 16 (odd nrandom; DXT1 blocks whose transparent texels leave
-    fewer than 16 colours), up to the GPU limit of 208, every format: the
+    fewer than 16 colours), up to the LDS-resident limit of 208, every format: the
     bounded distances (AVG, WAVG, W0AVG, NORMALMAP) and DXT5 alpha take the
     keyed pair scan, SRGB the per-step scan with its negative-sum rule."""
     rng = np.random.default_rng(300 + cd)
@@ -149,3 +149,28 @@ def test_random_candidate_counts(gpu, oracle, cd):
             for kind in ("speckle", "random", "extreme"):
                 q = imgutil.random_quantized(rng, w, h, dxt, kind)
                 _compare(gpu, oracle, q, w, h, dxt, cd, nr, 2 if nr % 2 else 1, 11 + nr, f"candidates-{kind}")
+
+
+@pytest.mark.parametrize("cd", [7, 2, 0, 5])
+def test_wide_candidate_counts(gpu, oracle, cd):
+    """More random candidates than LDS holds (nrandom 209 .. 700): the same
+    encoder with its candidates, distance rows and column features in device
+    scratch, on images of several workgroups' worth of blocks (the persistent
+    grid strides over them), every format; NORMALMAP and WAVG take the keyed
+    scan, RGB its 64-bit keys, SRGB the per-step scan with its negative-sum
+    rule."""
+    rng = np.random.default_rng(900 + cd)
+    for dxt in range(3):
+        for nr, (w, h) in ((209, (23, 17)), (255, (9, 6)), (497, (7, 5)), (700, (5, 4))):
+            q = imgutil.random_quantized(rng, w, h, dxt, "random" if nr % 2 else "speckle")
+            _compare(gpu, oracle, q, w, h, dxt, cd, nr, 2 if nr % 2 else 1, 3 + nr, "wide")
+
+
+def test_wide_grid_strides(gpu, oracle):
+    """A 264x260 image at nrandom 209 has 1,089 four-block tiles, more than
+    the wide kernel's grid of 4 workgroups per CU: workgroups take several
+    tiles each and reuse their scratch slice."""
+    rng = np.random.default_rng(77)
+    w, h = 264, 260
+    q = imgutil.random_quantized(rng, w, h, 0, "random")
+    _compare(gpu, oracle, q, w, h, 0, 7, 209, 2, 9, "wide-strided")

@@ -93,6 +93,33 @@ def test_cli_on_reference_test_images(gpu):
         assert got == digest, key
 
 
+def test_cli_sinus256_default_settings(gpu):
+    """The 256x256 sinusoid workload through this project's s2tc_compress
+    with the library defaults (no S2TC_* variables: SIMPLE dither, WAVG,
+    quick mode, REFINE_ALWAYS), DXT1, the whole mip chain in the DDS: the
+    reference CLI's file byte for byte.  The reference CLI loading this
+    library writes the same file."""
+    import imgutil
+    from s2tc_amd import workloads
+    wl = workloads.IMAGE_WORKLOADS["sinus256"]
+    exp = DIGESTS["cli_workloads"]["sinus256"]
+    src = workloads.make_image_source(wl)
+    env = {k: v for k, v in os.environ.items() if not k.startswith("S2TC_")}
+    with tempfile.TemporaryDirectory() as td:
+        t, o = os.path.join(td, "i.tga"), os.path.join(td, "o.dds")
+        imgutil.write_tga32(t, src, wl["width"], wl["height"])
+        assert hashlib.sha256(open(t, "rb").read()).hexdigest() == exp["tga_sha256"]
+        clis = [(os.path.join(ROOT, "s2tc_amd", "s2tc_compress"), None)]
+        ref_cli = os.path.join(ROOT, "oracle", "_ref", "s2tc_compress")
+        if os.path.exists(ref_cli):
+            clis.append((ref_cli, os.path.join(ROOT, "s2tc_amd", "libtxc_dxtn.so")))
+        for cli, lib in clis:
+            _say(f"sinus256 through {os.path.basename(os.path.dirname(cli))}/s2tc_compress, defaults")
+            cmd = [cli, "-i", t, "-o", o, "-t", "DXT1"] + (["-l", lib] if lib else [])
+            subprocess.run(cmd, check=True, env=env, capture_output=True, timeout=300)
+            assert hashlib.sha256(open(o, "rb").read()).hexdigest() == exp["dds_sha256"], cli
+
+
 def test_reference_cli_loads_this_library(gpu):
     """Drop-in check: the reference's own CLI (built from its sources) loads
     this project's libtxc_dxtn.so with -l and writes the reference's bytes."""

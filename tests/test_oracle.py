@@ -61,6 +61,20 @@ def test_oracle_matches_golden_compress(oracle):
         assert np.array_equal(got, z["dst_" + str(key)]), key
 
 
+def test_oracle_matches_golden_wide(oracle):
+    """Random-colour counts from 209 to 3000 (candidate lists far beyond 16
+    + 208): the oracle equals the reference's tx_compress_dxtn."""
+    z = load("wide.npz")
+    for key in z["cases"]:
+        w, h, fmt, cd, nr, refine = map(int, str(key).split("_"))
+        bs = 8 if fmt < 0x83F2 else 16
+        r, got = oracle.compress(4, w, h, z[f"src_{w}_{h}"], fmt, ((w + 3) // 4) * bs, cd, nr, refine, 0, seed=5,
+                                 fill=0xAB)
+        assert r == 0
+        exp = z["dst_" + str(key)]
+        assert np.array_equal(got[:exp.size], exp), key
+
+
 def test_oracle_matches_compiled_reference_live(oracle, reference):
     rng = np.random.default_rng(99)
     for (w, h), dxt, kind in itertools.product([(11, 6), (4, 4), (1, 2)], range(3),
