@@ -83,6 +83,19 @@ S2TC_API int s2tc_amd_memcpy_d2d_kernel(void *dst, const void *src, size_t bytes
 S2TC_API int s2tc_amd_encode_image(const void *d_src, int src_kind, int width, int height, void *d_dst,
                           int dxt, int cd, int nrandom, int refine, uint64_t seed, void *stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under stream capture, tight serving loops, hosts that forbid
+ * hipMalloc): s2tc_amd_encode_image with the same arguments and output,
+ * calling no hipMalloc/hipFree or allocating routine.  Random-candidate
+ * encodes above S2TC_LDS_NRANDOM take their group memory (4 * 68 bytes per
+ * candidate slot per workgroup) from a static device array of
+ * S2TC_NOALLOC_SCRATCH bytes that each GPU holds until the process ends,
+ * with as many workgroups as fit in it.  Calls that overlap on one GPU share
+ * that array and race: serialize them (one stream, or one host thread). */
+#define S2TC_NOALLOC_SCRATCH (64u << 20)
+S2TC_API int s2tc_amd_encode_image_noalloc(const void *d_src, int src_kind, int width, int height, void *d_dst,
+                                   int dxt, int cd, int nrandom, int refine, uint64_t seed, void *stream);
+
 /* ---- batches: many images (textures x mip levels) in one graph launch ---- */
 
 typedef struct {

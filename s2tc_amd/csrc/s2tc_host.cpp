@@ -10,6 +10,7 @@
 #include "plan.h"
 #include "s2tc_internal.h"
 
+#include <algorithm>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -106,7 +107,7 @@ int ensure_device()
 	return ensure_tables();
 }
 
-int launch_encode(const EncodeArgs &a, hipStream_t stream)
+int launch_encode(const EncodeArgs &a, hipStream_t stream, void *ext_scratch, size_t ext_bytes)
 {
 	int dxt = a.dxt, cd = a.cd, refine = a.refine;
 	normalize_modes(dxt, cd, refine);
@@ -164,18 +165,26 @@ int launch_encode(const EncodeArgs &a, hipStream_t stream)
 		dim3 grid((unsigned)((nb + 3) / 4)), block(64);
 		return hip_check(hipLaunchKernel(kt.normal[raw], grid, block, args, smem, stream), "launch k_normal");
 	}
-	// more candidates than LDS holds: group memory in stream-ordered scratch
-	// that lives exactly as long as this launch
+	// more candidates than LDS holds: group memory in the caller's scratch, or
+	// in stream-ordered scratch that lives exactly as long as this launch
 	int grid = 0;
 	size_t bytes = 0;
 	wide_launch_shape(kt, pbw, (nb + 3) / 4, device_cu_count(), grid, bytes);
-	void *gmem = nullptr;
-	if (hipMallocAsync(&gmem, bytes, stream) != hipSuccess)
+	void *gmem = ext_scratch;
+	if (ext_scratch) {
+		const size_t wg = bytes / (size_t)grid;
+		if (ext_bytes < wg)
+			return set_error(S2TC_E_RANGE, "nrandom %d needs %zu bytes of scratch per workgroup, %zu available",
+			                 nrandom, wg, ext_bytes);
+		grid = (int)std::min((size_t)grid, ext_bytes / wg);
+	} else if (hipMallocAsync(&gmem, bytes, stream) != hipSuccess) {
 		return set_error(S2TC_E_RANGE, "cannot allocate %zu bytes of scratch for nrandom %d", bytes, nrandom);
+	}
 	void *args[] = {(void *)&src, (void *)&width, (void *)&height, (void *)&dst, (void *)&dxt, (void *)&nrandom,
 	                (void *)&refine, (void *)&seed, (void *)&pbw, (void *)&block_base, (void *)&gmem};
 	const int rc = hip_check(hipLaunchKernel(kt.normal_wide[raw], dim3((unsigned)grid), dim3(64), args, 0, stream),
 	                         "launch k_normal_wide");
+	if (ext_scratch) return rc;
 	const int rc2 = hip_check(hipFreeAsync(gmem, stream), "hipFreeAsync (scratch)");
 	return rc ? rc : rc2;
 }

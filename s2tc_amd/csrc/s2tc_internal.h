@@ -37,7 +37,11 @@ const KernelTable &kernel_table(int cd);
 void normalize_modes(int &dxt, int &cd, int &refine);
 bool is_fast(int cd, int nrandom);
 int normal_pbw(int nrandom);
-int launch_encode(const EncodeArgs &a, hipStream_t stream);
+// Encodes one image on `stream`.  Random-candidate encodes above
+// S2TC_LDS_NRANDOM need device scratch: from `ext_scratch` (ext_bytes long;
+// the grid shrinks to what fits) when given, else allocated stream-ordered
+// around the launch.
+int launch_encode(const EncodeArgs &a, hipStream_t stream, void *ext_scratch = nullptr, size_t ext_bytes = 0);
 
 // rgb565_image on the device (dither.hip); d_scratch holds rgb565_scratch_bytes(w, h)
 size_t rgb565_scratch_bytes(int w, int h);
