@@ -146,12 +146,18 @@ def test_full_batch_matches_reference_digest(gpu):
     items = imgs.items
     assert len(items) == exp["jobs"] and imgs.texels == exp["texels"]
     b = gpu.Batch(imgs)
-    _say("full batch launch")
-    b.launch()
-    gpu.synchronize()
-    host = imgs.dst.to_host()
-    hashes = b"".join(hashlib.sha256(imgs.output(i, host).tobytes()).digest() for i in range(len(items)))
-    assert hashlib.sha256(hashes).hexdigest() == exp["sha256_of_job_sha256s"]
+    # three launches with the outputs cleared in between and different
+    # schedules: every one must reproduce the reference bytes (a check for
+    # run-to-run variation as well as for parity)
+    for rep, sched in enumerate([(4, 4), (8, 8), (1, 2)]):
+        b.set_schedule(*sched)
+        imgs.dst.fill(0)
+        _say(f"full batch launch {rep}, schedule {sched}")
+        b.launch()
+        gpu.synchronize()
+        host = imgs.dst.to_host()
+        hashes = b"".join(hashlib.sha256(imgs.output(i, host).tobytes()).digest() for i in range(len(items)))
+        assert hashlib.sha256(hashes).hexdigest() == exp["sha256_of_job_sha256s"], (rep, sched)
     b.destroy()
     imgs.free()
 
