@@ -243,21 +243,6 @@ __global__ void k_quantize_none(const uint8_t *__restrict__ src, int srccomps, l
 	}
 }
 
-__device__ __forceinline__ int floyd_q(int src, int acc, int shift, int maxval, int &err)
-{
-	const int s = ((src << 4) | (src >> 4)) + acc;
-	int r = s >> (shift + 4);
-	r = r < 0 ? 0 : (r > maxval ? maxval : r);
-	err = s - r * 4095 / maxval;
-	return r;
-}
-__device__ __forceinline__ int floyd1_q(int src, int acc, int &err)
-{
-	const int s = ((src << 4) | (src >> 4)) + acc;
-	const int r = s >= 2048 ? 1 : 0;
-	err = s - (r ? 4095 : 0);
-	return r;
-}
 // e7, e3, e5, e1 of floyd/floyd1 (:1227-1233), C division (toward zero)
 __device__ __forceinline__ void fs_split(int err, int &e7, int &e3, int &e5, int &e1)
 {
@@ -292,69 +277,88 @@ __device__ __forceinline__ void fs_store(int *p, int v)
 	__hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// One Floyd-Steinberg pass over NCH independent channels.  carry: NCH*(w+2)
-// ints of scratch.  init0 (w+2 ints, may be null) is A_0 of channel 0, the
-// errors its first row starts from (zero otherwise).  rleft (w+2 ints, may
-// be null) receives what the reference's buffer holds in the first w+2 ints
-// after the pass, for channel 0: the accumulated errors the last row read
+// One Floyd-Steinberg pass per workgroup: workgroup k runs channel k
+// (c0, c1, c2) with its own carry row at carry + k * (w + 2); the channels
+// of the reference's colour pass are independent (separate error rows,
+// s2tc_algorithm.cpp:1356-1372), so they run side by side on different CUs.
+// init0 (w+2 ints, may be null) is A_0 of workgroup 0's channel, the errors
+// its first row starts from (zero otherwise).  rleft (w+2 ints, may be null)
+// receives what the reference's buffer holds in the first w+2 ints after the
+// pass, for workgroup 0's channel: the accumulated errors the last row read
 // (thisrow, when h-1 is even) or that row's downward errors (downrow, when
-// h-1 is odd).
-template <int NCH, int FS_R>
+// h-1 is odd).  The quantized step divides by maxval (15, 31 or 63) with one
+// high multiply: floor(v / d) = mul_hi(v, floor(2^32 / d) + 1) for v < 2^20
+// and odd d < 2^12, since the product overshoots v / d by less than
+// 2^-12 < 1 / d.
+template <int FS_R>
 __global__ __launch_bounds__(FS_R) void k_dither_fs(const uint8_t *__restrict__ src, int srccomps, int w, int h,
                                                     FsChan c0, FsChan c1, FsChan c2, const int *__restrict__ init0,
-                                                    int *__restrict__ carry, int *__restrict__ rleft,
+                                                    int *__restrict__ carry_all, int *__restrict__ rleft_all,
                                                     uint8_t *__restrict__ out)
 {
-	__shared__ int slot[NCH][FS_R][2];
-	const FsChan chs[3] = {c0, c1, c2};
+	__shared__ int slot[FS_R][2];
+	const int k = (int)blockIdx.x;
+	const FsChan ch = k == 0 ? c0 : k == 1 ? c1 : c2;
 	const int r = (int)threadIdx.x;
 	const int pw = w + 2;
-	for (int j = r; j < pw; j += FS_R) {
-		fs_store(&carry[j], init0 ? init0[j] : 0);
-		for (int c = 1; c < NCH; ++c) fs_store(&carry[c * pw + j], 0);
-	}
+	int *carry = carry_all + (size_t)k * pw;
+	const int *init = k == 0 ? init0 : nullptr;
+	int *rleft = k == 0 ? rleft_all : nullptr;
+	const int maxval = (1 << (8 - ch.shift)) - 1;
+	const uint32_t rec = 0xffffffffu / (uint32_t)maxval + 1u;
+	for (int j = r; j < pw; j += FS_R) fs_store(&carry[j], init ? init[j] : 0);
 	__syncthreads();
 	for (int y0 = 0; y0 < h; y0 += FS_R) {
 		const int rows = min(FS_R, h - y0);
 		const int y = y0 + r;
 		const bool active = r < rows;
-		int right[NCH], win1[NCH], win2[NCH];
-#pragma unroll
-		for (int c = 0; c < NCH; ++c) right[c] = win1[c] = win2[c] = 0;
+		const uint8_t *srow = src + (size_t)(active ? y : 0) * (size_t)w * (size_t)srccomps + ch.off;
+		uint8_t *orow = out + (size_t)(active ? y : 0) * (size_t)w * 4 + ch.off;
+		int right = 0, win1 = 0, win2 = 0;
 		const int steps = w + 2 * (rows - 1) + 1;
 		for (int t = 0; t < steps; ++t) {
 			const int x = t - 2 * r;
 			if (active && x >= 0 && x <= w) {
-#pragma unroll
-				for (int c = 0; c < NCH; ++c) {
-					const FsChan ch = chs[c];
-					int fin;
-					if (x < w) {
-						const int above = r == 0 ? fs_load(&carry[c * pw + x + 1]) : slot[c][r - 1][(t - 1) & 1];
-						const int acc = above + right[c];
-						const int sv = src[((size_t)y * w + x) * srccomps + ch.off];
-						int err, q;
-						q = ch.kind == 1 ? floyd1_q(sv, acc, err)
-						                 : floyd_q(sv, acc, ch.shift, (1 << (8 - ch.shift)) - 1, err);
-						out[((size_t)y * w + x) * 4 + ch.off] = (uint8_t)q;
-						int e7, e3, e5, e1;
-						fs_split(err, e7, e3, e5, e1);
-						right[c] = e7;
-						fin = win1[c] + e3;
-						win1[c] = win2[c] + e5;
-						win2[c] = e1;
-						if (c == 0 && rleft && y == h - 1 && ((h - 1) & 1) == 0) rleft[x + 1] = acc;
+				int fin;
+				if (x < w) {
+					const int above = r == 0 ? fs_load(&carry[x + 1]) : slot[r - 1][(t - 1) & 1];
+					const int acc = above + right;
+					const int sv = srow[x * srccomps];
+					const int v = ((sv << 4) | (sv >> 4)) + acc;
+					int q, err;
+					if (ch.kind == 1) {
+						q = v >= 2048 ? 1 : 0;
+						err = v - (q ? 4095 : 0);
 					} else {
-						fin = win1[c];
+						q = v >> (ch.shift + 4);
+						q = q < 0 ? 0 : (q > maxval ? maxval : q);
+						err = v - (int)__umulhi((uint32_t)(q * 4095), rec);
 					}
-					if (r + 1 < rows) slot[c][r][t & 1] = fin;  // A_{y+1}[x] for row y+1
-					else fs_store(&carry[c * pw + x], fin);
-					if (c == 0 && rleft && y == h - 1 && ((h - 1) & 1) == 1) rleft[x] = fin;
+					orow[x * 4] = (uint8_t)q;
+					int e7, e3, e5, e1;
+					fs_split(err, e7, e3, e5, e1);
+					right = e7;
+					fin = win1 + e3;
+					win1 = win2 + e5;
+					win2 = e1;
+					if (rleft && y == h - 1 && ((h - 1) & 1) == 0) rleft[x + 1] = acc;
+				} else {
+					fin = win1;
 				}
+				if (r + 1 < rows) slot[r][t & 1] = fin;  // A_{y+1}[x] for row y+1
+				else fs_store(&carry[x], fin);
+				if (rleft && y == h - 1 && ((h - 1) & 1) == 1) rleft[x] = fin;
 			}
 			__syncthreads();
 		}
 	}
+}
+
+template <int FS_R>
+static void fs_pass(int nch, const uint8_t *src, int srccomps, int w, int h, FsChan c0, FsChan c1, FsChan c2,
+                    const int *init0, int *carry, int *rleft, uint8_t *out, hipStream_t st)
+{
+	k_dither_fs<FS_R><<<nch, FS_R, 0, st>>>(src, srccomps, w, h, c0, c1, c2, init0, carry, rleft, out);
 }
 
 static int launch_rgb565_fs(const uint8_t *src, int srccomps, int w, int h, int alphabits, uint8_t *out,
@@ -366,15 +370,17 @@ static int launch_rgb565_fs(const uint8_t *src, int srccomps, int w, int h, int 
 	const FsChan cr{0, 0, 3}, cg{1, 0, 2}, cb{2, 0, 3};
 	const int R = h <= 256 ? 256 : h <= 512 ? 512 : 1024;
 	int *rl = alpha_fs ? rleft : nullptr;
-	if (R == 256) k_dither_fs<3, 256><<<1, 256, 0, st>>>(src, srccomps, w, h, cr, cg, cb, nullptr, carry, rl, out);
-	else if (R == 512) k_dither_fs<3, 512><<<1, 512, 0, st>>>(src, srccomps, w, h, cr, cg, cb, nullptr, carry, rl, out);
-	else k_dither_fs<3, 1024><<<1, 1024, 0, st>>>(src, srccomps, w, h, cr, cg, cb, nullptr, carry, rl, out);
+	// r, g and b: three workgroups, one channel each
+	if (R == 256) fs_pass<256>(3, src, srccomps, w, h, cr, cg, cb, nullptr, carry, rl, out, st);
+	else if (R == 512) fs_pass<512>(3, src, srccomps, w, h, cr, cg, cb, nullptr, carry, rl, out, st);
+	else fs_pass<1024>(3, src, srccomps, w, h, cr, cg, cb, nullptr, carry, rl, out, st);
 	if (int r = hip_check(hipGetLastError(), "dither FS colour kernel")) return r;
 	if (alpha_fs) {
+		// alpha starts from what the r pass left in the shared buffer
 		const FsChan ca{3, alphabits == 1 ? 1 : 0, 8 - alphabits};
-		if (R == 256) k_dither_fs<1, 256><<<1, 256, 0, st>>>(src, srccomps, w, h, ca, ca, ca, rleft, carry, nullptr, out);
-		else if (R == 512) k_dither_fs<1, 512><<<1, 512, 0, st>>>(src, srccomps, w, h, ca, ca, ca, rleft, carry, nullptr, out);
-		else k_dither_fs<1, 1024><<<1, 1024, 0, st>>>(src, srccomps, w, h, ca, ca, ca, rleft, carry, nullptr, out);
+		if (R == 256) fs_pass<256>(1, src, srccomps, w, h, ca, ca, ca, rleft, carry, nullptr, out, st);
+		else if (R == 512) fs_pass<512>(1, src, srccomps, w, h, ca, ca, ca, rleft, carry, nullptr, out, st);
+		else fs_pass<1024>(1, src, srccomps, w, h, ca, ca, ca, rleft, carry, nullptr, out, st);
 		return hip_check(hipGetLastError(), "dither FS alpha kernel");
 	}
 	const long long n = (long long)w * h;

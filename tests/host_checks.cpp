@@ -26,8 +26,24 @@ static bool slices_cover(size_t len, unsigned nt)
 	return next == len;
 }
 
+// The Floyd-Steinberg step divides v = q * 4095 (q <= maxval) by maxval in
+// {15, 31, 63} as mul_hi(v, floor(2^32 / d) + 1); exact for every v < 2^20.
+static bool fs_mulhi_division_exact()
+{
+	for (unsigned d : {15u, 31u, 63u}) {
+		const unsigned rec = 0xffffffffu / d + 1u;
+		for (unsigned v = 0; v < (1u << 20); ++v)
+			if ((unsigned)(((unsigned long long)v * rec) >> 32) != v / d) {
+				printf("fs division: d %u v %u\n", d, v);
+				return false;
+			}
+	}
+	return true;
+}
+
 int main()
 {
+	if (!fs_mulhi_division_exact()) return 1;
 	std::vector<size_t> lens;
 	for (size_t n = 0; n < 2048; ++n) lens.push_back(n);
 	// chunk tails of large uploads: multiples of 64 * nt plus a few bytes,
