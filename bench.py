@@ -519,11 +519,14 @@ def class_roofline(s, c, imgs, nclasses):
     n = 16 texel colours and m = 16 + nrandom candidates, plus the same again
     for the DXT5 alpha scan of every block that runs one (full blocks with a
     single alpha value run none) - and its fraction of the VALU peak."""
-    # a batch of one all-16 class runs the two-waves-per-SIMD form of the
-    # kernel for every distance but AVG (4), WAVG (5), W0AVG (6)
-    alone = nclasses == 1 and c["normal"] == 2 and c["cd"] not in (4, 5, 6)
+    # all-16 classes run the kernel form the autotuner chose: one lane per
+    # block with one (k_all16_batch) or two (k_all16_batch_alone) waves per
+    # SIMD, or 16 lanes per block (k_normal_batch)
+    all16 = {0: f"k_all16_batch<{c['cd']}, {c['dxt']}, {c['refine']}>",
+             1: f"k_all16_batch_alone<{c['cd']}, {c['dxt']}, {c['refine']}>",
+             2: f"k_normal_batch<{c['cd']}>"}[c["form"]]
     c["kernel_name"] = {0: f"k_fast_batch<{c['cd']}, {c['dxt']}, {c['refine']}>", 1: f"k_normal_batch<{c['cd']}>",
-                        2: f"k_all16_batch{'_alone' if alone else ''}<{c['cd']}, {c['dxt']}, {c['refine']}>"}[c["normal"]]
+                        2: all16}[c["normal"]]
     c["gbs"] = c["bytes"] / (c["ms"] * 1e-3) / 1e9
     c["hbm_frac"] = c["gbs"] / HBM_PEAK_GBS
     ops = 0
@@ -686,7 +689,9 @@ def main():
     roofline.update(traffic=sum(tr) if tr else None, kernel=kname, kernel_ms=round(dom["ms"], 4))
     if tr:
         roofline["traffic_source"] = "profiles/traffic_per_launch.txt (rocprofv3 FETCH_SIZE x2, WRITE_SIZE x1)"
-    class_summary = [dict(kernel=c["kernel_name"], cd=names[c["cd"]], dxt=c["dxt"], refine=c["refine"],
+    form_names = {0: "lane", 1: "lane_2waves", 2: "row16"}
+    class_summary = [dict(kernel=c["kernel_name"], form=form_names[c["form"]], cd=names[c["cd"]], dxt=c["dxt"],
+                          refine=c["refine"],
                           jobs=c["njobs"], texels=c["texels"], ms=round(c["ms"], 4), gbs=round(c["gbs"], 1),
                           hbm_frac=round(c["hbm_frac"], 4),
                           valu_frac=round(c["valu_frac"], 4) if c["valu_ops"] else None) for c in classes]

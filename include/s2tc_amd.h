@@ -117,7 +117,15 @@ typedef struct {
 	long long blocks, texels;
 	long long bytes;   /* algorithmic HBM bytes: 4 per texel read + block bytes written */
 	double ms;         /* kernel time of one launch */
+	int form;          /* kernel form (S2TC_FORM_*) the class runs */
 } s2tc_amd_class_time;
+
+/* Kernel forms of a batch class.  Quick and random-candidate classes have
+ * one form (S2TC_FORM_LANE and S2TC_FORM_ROW16); classes of MODE_NORMAL
+ * with the block colours only have up to three. */
+#define S2TC_FORM_LANE        0   /* one lane per block, one wave per SIMD */
+#define S2TC_FORM_LANE_2WAVES 1   /* one lane per block, two waves per SIMD (not for AVG, WAVG, W0AVG) */
+#define S2TC_FORM_ROW16       2   /* 16 lanes (one DPP row) per block, four blocks per wave */
 
 typedef struct s2tc_amd_batch s2tc_amd_batch;
 
@@ -132,9 +140,16 @@ S2TC_API int s2tc_amd_batch_create(const s2tc_amd_job *jobs, int njobs, s2tc_amd
  * per tile, i.e. not persistent) and the number of streams the classes are
  * spread over inside the graph.  Re-captures the graph. */
 S2TC_API int s2tc_amd_batch_set_schedule(s2tc_amd_batch *b, int wg_per_cu, int streams);
-/* Times every schedule of a small grid (reps graph launches each) and keeps
- * the fastest; *best_ms gets its time per launch. */
+/* Autotuner: times whole-graph launches (reps each) over a grid of
+ * schedules, then over the kernel forms of each MODE_NORMAL class with the
+ * block colours only (classes in decreasing cost, each form with the others
+ * fixed), then over the schedule grid again, and keeps the fastest;
+ * *best_ms gets its time per launch. */
 S2TC_API int s2tc_amd_batch_autotune(s2tc_amd_batch *b, void *stream, int reps, double *best_ms);
+/* Sets the kernel form of class `klass` (the order of
+ * s2tc_amd_batch_class_times) and re-captures the graph; S2TC_E_ARG if the
+ * class has no such form. */
+S2TC_API int s2tc_amd_batch_set_form(s2tc_amd_batch *b, int klass, int form);
 S2TC_API int s2tc_amd_batch_get_schedule(const s2tc_amd_batch *b, int *wg_per_cu, int *streams);
 /* One launch of the whole batch (graph replay), asynchronous on stream. */
 S2TC_API int s2tc_amd_batch_launch(s2tc_amd_batch *b, void *stream);

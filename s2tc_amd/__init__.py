@@ -280,7 +280,12 @@ class Job(ctypes.Structure):
 class ClassTime(ctypes.Structure):
     """s2tc_amd_class_time"""
     _fields_ = [("normal", _I), ("cd", _I), ("dxt", _I), ("refine", _I), ("njobs", _I), ("blocks", ctypes.c_longlong),
-                ("texels", ctypes.c_longlong), ("bytes", ctypes.c_longlong), ("ms", ctypes.c_double)]
+                ("texels", ctypes.c_longlong), ("bytes", ctypes.c_longlong), ("ms", ctypes.c_double),
+                ("form", _I)]
+
+
+# kernel forms of a batch class (s2tc_amd.h S2TC_FORM_*)
+FORM_LANE, FORM_LANE_2WAVES, FORM_ROW16 = 0, 1, 2
 
 
 def _bind_batch(L):
@@ -290,6 +295,7 @@ def _bind_batch(L):
     L.s2tc_amd_batch_set_schedule.argtypes = [_P, _I, _I]
     L.s2tc_amd_batch_get_schedule.argtypes = [_P, ctypes.POINTER(_I), ctypes.POINTER(_I)]
     L.s2tc_amd_batch_autotune.argtypes = [_P, _P, _I, ctypes.POINTER(ctypes.c_double)]
+    L.s2tc_amd_batch_set_form.argtypes = [_P, _I, _I]
     L.s2tc_amd_batch_launch.argtypes = [_P, _P]
     L.s2tc_amd_batch_class_times.argtypes = [_P, _P, _P, _I]
     L.s2tc_amd_batch_destroy.argtypes = [_P]
@@ -418,6 +424,10 @@ class Batch:
         check(lib().s2tc_amd_batch_autotune(self.h, stream, reps, ctypes.byref(ms)))
         return ms.value
 
+    def set_form(self, klass, form):
+        """Kernel form of class `klass` (index into class_times())."""
+        check(lib().s2tc_amd_batch_set_form(self.h, klass, form))
+
     def class_times(self, stream=None):
         arr = (ClassTime * 64)()
         n = lib().s2tc_amd_batch_class_times(self.h, stream, ctypes.cast(arr, _P), 64)
@@ -425,7 +435,7 @@ class Batch:
             check(n)
         return [dict(normal=c.normal, cd=c.cd, dxt=c.dxt, refine=c.refine, njobs=c.njobs, blocks=c.blocks,
                      texels=c.texels,
-                     bytes=c.bytes, ms=c.ms) for c in arr[:n]]
+                     bytes=c.bytes, ms=c.ms, form=c.form) for c in arr[:n]]
 
     def destroy(self):
         if self.h:

@@ -91,6 +91,92 @@ static void free_plan(s2tc_amd_batch *b)
 
 }  // namespace s2tc
 
+namespace s2tc {
+
+// Whole-graph timing of the batch's current plan: capture, one warm-up
+// launch, then reps launches between two events.
+struct GraphTimer {
+	s2tc_amd_batch *b;
+	hipStream_t st;
+	int reps;
+	hipEvent_t e0 = nullptr, e1 = nullptr;
+	int init()
+	{
+		if (int r = hip_check(hipEventCreate(&e0), "hipEventCreate")) return r;
+		return hip_check(hipEventCreate(&e1), "hipEventCreate");
+	}
+	~GraphTimer()
+	{
+		if (e0) (void)hipEventDestroy(e0);
+		if (e1) (void)hipEventDestroy(e1);
+	}
+	int time(double &per)
+	{
+		if (int r = capture(b)) return r;
+		if (int r = s2tc_amd_batch_launch(b, st)) return r;  // warm-up
+		if (int r = hip_check(hipEventRecord(e0, st), "hipEventRecord")) return r;
+		for (int i = 0; i < reps; ++i)
+			if (int r = s2tc_amd_batch_launch(b, st)) return r;
+		if (int r = hip_check(hipEventRecord(e1, st), "hipEventRecord")) return r;
+		if (int r = hip_check(hipEventSynchronize(e1), "hipEventSynchronize")) return r;
+		float ms = 0;
+		(void)hipEventElapsedTime(&ms, e0, e1);
+		per = ms / reps;
+		return 0;
+	}
+};
+
+// The schedule grid: workgroups per CU x capture streams; keeps the fastest.
+static int tune_schedule(s2tc_amd_batch *b, GraphTimer &gt, double &best)
+{
+	static const int kWg[] = {1, 2, 4, 8, 16, 0};
+	static const int kStreams[] = {1, 2, 4, 8};
+	int bw = b->wg_per_cu, bs = b->streams;
+	for (int wg : kWg)
+		for (int ns : kStreams) {
+			b->wg_per_cu = wg;
+			b->streams = ns;
+			double per = 0;
+			if (int r = gt.time(per)) return r;
+			if (per < best) {
+				best = per;
+				bw = wg;
+				bs = ns;
+			}
+		}
+	b->wg_per_cu = bw;
+	b->streams = bs;
+	return 0;
+}
+
+// The kernel forms of each all-16 class, one class at a time in decreasing
+// cost with the others fixed; keeps a form only if the whole graph gets
+// faster.
+static int tune_forms(s2tc_amd_batch *b, GraphTimer &gt, double &best)
+{
+	for (ClassPlan &c : b->classes) {
+		if (!c.all16) continue;
+		int keep = c.form;
+		for (int f = 0; f < 3; ++f) {
+			if (f == keep || !c.forms[f]) continue;
+			c.form = f;
+			double per = 0;
+			if (int r = gt.time(per)) {
+				c.form = keep;
+				return r;
+			}
+			if (per < best) {
+				best = per;
+				keep = f;
+			}
+		}
+		c.form = keep;
+	}
+	return 0;
+}
+
+}  // namespace s2tc
+
 using namespace s2tc;
 
 extern "C" {
@@ -102,7 +188,7 @@ int s2tc_amd_batch_create(const s2tc_amd_job *jobs, int njobs, s2tc_amd_batch **
 	if (int r = ensure_device()) return r;
 	s2tc_amd_batch *b = new s2tc_amd_batch;
 	b->ncu = device_cu_count();
-	int rc = plan_classes(jobs, njobs, b->classes);
+	int rc = plan_classes(jobs, njobs, b->classes, true);
 	if (!rc) {
 		const size_t bytes = plan_descriptor_bytes(b->classes);
 		std::vector<uint8_t> host(bytes);
@@ -163,51 +249,32 @@ int s2tc_amd_batch_launch(s2tc_amd_batch *b, void *stream)
 	return hip_check(hipGraphLaunch(b->exec, (hipStream_t)stream), "hipGraphLaunch");
 }
 
+
 int s2tc_amd_batch_autotune(s2tc_amd_batch *b, void *stream, int reps, double *best_ms)
 {
 	if (!b) return set_error(S2TC_E_ARG, "null batch");
-	if (reps < 1) reps = 1;
-	hipStream_t st = (hipStream_t)stream;
-	hipEvent_t e0, e1;
-	if (int r = hip_check(hipEventCreate(&e0), "hipEventCreate")) return r;
-	if (int r = hip_check(hipEventCreate(&e1), "hipEventCreate")) {
-		(void)hipEventDestroy(e0);
-		return r;
-	}
-	static const int kWg[] = {1, 2, 4, 8, 16, 0};
-	static const int kStreams[] = {1, 2, 4, 8};
+	GraphTimer gt{b, (hipStream_t)stream, reps < 1 ? 1 : reps};
+	int rc = gt.init();
 	double best = 1e30;
-	int bw = b->wg_per_cu, bs = b->streams, rc = 0;
-	for (int wg : kWg) {
-		for (int ns : kStreams) {
-			b->wg_per_cu = wg;
-			b->streams = ns;
-			if ((rc = capture(b))) break;
-			if ((rc = s2tc_amd_batch_launch(b, stream))) break;  // warm-up
-			if ((rc = hip_check(hipEventRecord(e0, st), "hipEventRecord"))) break;
-			for (int i = 0; i < reps && !rc; ++i) rc = s2tc_amd_batch_launch(b, stream);
-			if (!rc) rc = hip_check(hipEventRecord(e1, st), "hipEventRecord");
-			if (!rc) rc = hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
-			if (rc) break;
-			float ms = 0;
-			(void)hipEventElapsedTime(&ms, e0, e1);
-			const double per = ms / reps;
-			if (per < best) {
-				best = per;
-				bw = wg;
-				bs = ns;
-			}
-		}
-		if (rc) break;
-	}
-	(void)hipEventDestroy(e0);
-	(void)hipEventDestroy(e1);
-	b->wg_per_cu = bw;
-	b->streams = bs;
+	if (!rc) rc = gt.time(best);  // the current plan is the first candidate
+	if (!rc) rc = tune_schedule(b, gt, best);
+	if (!rc) rc = tune_forms(b, gt, best);
+	if (!rc) rc = tune_schedule(b, gt, best);
 	if (int r = capture(b)) return r;
 	if (rc) return rc;
 	if (best_ms) *best_ms = best;
 	return 0;
+}
+
+int s2tc_amd_batch_set_form(s2tc_amd_batch *b, int klass, int form)
+{
+	if (!b || klass < 0 || klass >= (int)b->classes.size()) return set_error(S2TC_E_ARG, "bad class index");
+	ClassPlan &c = b->classes[(size_t)klass];
+	const int native = c.normal ? FORM_ROW16 : FORM_LANE;
+	if (form < 0 || form > 2 || (c.all16 ? !c.forms[form] : form != native))
+		return set_error(S2TC_E_ARG, "class %d has no kernel form %d", klass, form);
+	if (c.all16) c.form = form;
+	return capture(b);
 }
 
 int s2tc_amd_batch_class_times(s2tc_amd_batch *b, void *stream, s2tc_amd_class_time *out, int max_out)
@@ -241,6 +308,7 @@ int s2tc_amd_batch_class_times(s2tc_amd_batch *b, void *stream, s2tc_amd_class_t
 			o.texels = c.texels;
 			o.bytes = c.bytes;
 			o.ms = ms;
+			o.form = c.all16 ? c.form : c.normal ? FORM_ROW16 : FORM_LANE;
 		}
 		++n;
 	}

@@ -20,7 +20,24 @@ int device_cu_count()
 	return ncu;
 }
 
-int plan_classes(const s2tc_amd_job *jobs, int njobs, std::vector<ClassPlan> &classes)
+// tile -> (first job, last job) of tiles of `tile` packed blocks, by a walk
+// over the jobs' first blocks
+static void tile_ranges(const std::vector<JobDesc> &jobs, long long nblocks, int tile, int ntiles,
+                        std::vector<TileJobs> &out)
+{
+	out.resize((size_t)ntiles);
+	int jlo = 0;
+	const int nj = (int)jobs.size();
+	for (int t = 0; t < ntiles; ++t) {
+		const long long b0 = (long long)t * tile, b1 = std::min(nblocks, b0 + tile) - 1;
+		while (jlo + 1 < nj && jobs[jlo + 1].block0 <= b0) ++jlo;
+		int jhi = jlo;
+		while (jhi + 1 < nj && jobs[jhi + 1].block0 <= b1) ++jhi;
+		out[(size_t)t] = TileJobs{jlo, jhi};
+	}
+}
+
+int plan_classes(const s2tc_amd_job *jobs, int njobs, std::vector<ClassPlan> &classes, bool with_forms)
 {
 	classes.clear();
 	// class slots: 0..71 quick ((cd*3 + dxt)*3 + refine), 72..79 MODE_NORMAL
@@ -91,21 +108,20 @@ int plan_classes(const s2tc_amd_job *jobs, int njobs, std::vector<ClassPlan> &cl
 		c.ntiles = (int)((c.nblocks + c.tile - 1) / c.tile);
 		c.smem = c.normal && !c.wide ? (size_t)kt.normal_smem_per_cand * 16 * c.pbw_max + (size_t)kt.normal_smem_fixed : 0;
 		if (c.wide) wide_launch_shape(kt, c.pbw_max, c.ntiles, device_cu_count(), c.wide_grid, c.scratch);
-		// tile -> (first job, last job), by a walk over the packed blocks
-		c.tiles.resize((size_t)c.ntiles);
-		int jlo = 0;
-		const int nj = (int)c.jobs.size();
-		for (int t = 0; t < c.ntiles; ++t) {
-			const long long b0 = (long long)t * c.tile, b1 = std::min(c.nblocks, b0 + c.tile) - 1;
-			while (jlo + 1 < nj && c.jobs[jlo + 1].block0 <= b0) ++jlo;
-			int jhi = jlo;
-			while (jhi + 1 < nj && c.jobs[jhi + 1].block0 <= b1) ++jhi;
-			c.tiles[(size_t)t] = TileJobs{jlo, jhi};
+		tile_ranges(c.jobs, c.nblocks, c.tile, c.ntiles, c.tiles);
+		if (c.all16) {
+			c.forms[FORM_LANE] = c.kernel;
+			const void *k2 = kt.all16_batch_alone[c.dxt][c.refine];
+			c.forms[FORM_LANE_2WAVES] = k2 != c.kernel ? k2 : nullptr;
+			if (with_forms) {
+				// the random-candidate kernel runs nrandom-0 jobs as they are
+				c.forms[FORM_ROW16] = kt.normal_batch;
+				c.ntiles4 = (int)((c.nblocks + 3) / 4);
+				c.smem4 = (size_t)kt.normal_smem_per_cand * 16 + (size_t)kt.normal_smem_fixed;
+				tile_ranges(c.jobs, c.nblocks, 4, c.ntiles4, c.tiles4);
+			}
 		}
 	}
-	// a lone all-16 class has the GPU to itself: its two-waves-per-SIMD form
-	if (classes.size() == 1 && classes[0].all16)
-		classes[0].kernel = kernel_table(classes[0].cd).all16_batch_alone[classes[0].dxt][classes[0].refine];
 	std::stable_sort(classes.begin(), classes.end(),
 	                 [](const ClassPlan &x, const ClassPlan &y) { return x.cost > y.cost; });
 	return 0;
@@ -117,7 +133,8 @@ size_t plan_descriptor_bytes(const std::vector<ClassPlan> &classes)
 {
 	size_t n = 0;
 	for (const auto &c : classes)
-		n += align16(c.jobs.size() * sizeof(JobDesc)) + align16(c.tiles.size() * sizeof(TileJobs));
+		n += align16(c.jobs.size() * sizeof(JobDesc)) + align16(c.tiles.size() * sizeof(TileJobs)) +
+		     align16(c.tiles4.size() * sizeof(TileJobs));
 	return n;
 }
 
@@ -132,6 +149,10 @@ void plan_pack(std::vector<ClassPlan> &classes, uint8_t *host, uint8_t *dev)
 		memcpy(host + off, c.tiles.data(), tb);
 		c.d_tiles = (TileJobs *)(dev + off);
 		off += align16(tb);
+		const size_t t4 = c.tiles4.size() * sizeof(TileJobs);
+		if (t4) memcpy(host + off, c.tiles4.data(), t4);
+		c.d_tiles4 = t4 ? (TileJobs *)(dev + off) : nullptr;
+		off += align16(t4);
 	}
 }
 
@@ -177,13 +198,23 @@ int plan_launch_class(const ClassPlan &c, int wg_per_cu, int ncu, hipStream_t st
 		return hip_check(hipLaunchKernel(c.kernel, dim3(c.wide_grid), dim3(64), args, 0, st),
 		                 "launch k_normal_batch_wide");
 	}
+	if (c.all16 && c.form == FORM_ROW16) {
+		int pbw = 1, nt4 = c.ntiles4;
+		const TileJobs *t4 = c.d_tiles4;
+		if (!t4 || !c.forms[FORM_ROW16]) return set_error(S2TC_E_ARG, "class has no 16-lane form");
+		const int grid4 = wg_per_cu <= 0 ? nt4 : std::min(nt4, 4 * wg_per_cu * ncu);
+		void *args[] = {(void *)&jobs, (void *)&t4, (void *)&nb, (void *)&nt4, (void *)&pbw};
+		return hip_check(hipLaunchKernel(c.forms[FORM_ROW16], dim3(grid4), dim3(64), args, c.smem4, st),
+		                 "launch k_normal_batch (all-16 jobs)");
+	}
 	if (c.normal) {
 		int pbw = c.pbw_max;
 		void *args[] = {(void *)&jobs, (void *)&tiles, (void *)&nb, (void *)&ntiles, (void *)&pbw};
 		return hip_check(hipLaunchKernel(c.kernel, dim3(grid), dim3(64), args, c.smem, st), "launch k_normal_batch");
 	}
 	void *args[] = {(void *)&jobs, (void *)&tiles, (void *)&nb, (void *)&ntiles};
-	return hip_check(hipLaunchKernel(c.kernel, dim3(grid), dim3(256), args, 0, st),
+	const void *k = c.all16 && c.forms[c.form] ? c.forms[c.form] : c.kernel;
+	return hip_check(hipLaunchKernel(k, dim3(grid), dim3(256), args, 0, st),
 	                 c.all16 ? "launch k_all16_batch" : "launch k_fast_batch");
 }
 

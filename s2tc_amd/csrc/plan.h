@@ -17,6 +17,14 @@ namespace s2tc {
 //     k_normal_batch, 16 lanes per block, tiles of 4 blocks; jobs with more
 //     than S2TC_LDS_NRANDOM random colours form their own class per distance
 //     (k_normal_batch_wide, group memory in device scratch).
+// Kernel forms of a class (s2tc_amd_batch_set_form): quick and
+// random-candidate classes have one; all-16 classes have up to three:
+enum {
+	FORM_LANE = 0,        // one lane per block, one wave per SIMD (k_all16_batch)
+	FORM_LANE_2WAVES = 1, // one lane per block, two waves per SIMD (k_all16_batch_alone)
+	FORM_ROW16 = 2,       // 16 lanes (one DPP row) per block, 4 blocks per wave (k_normal_batch)
+};
+
 struct ClassPlan {
 	bool normal = false;         // 16-lanes-per-block kernel
 	bool wide = false;           // ... with its group memory in device scratch
@@ -33,15 +41,25 @@ struct ClassPlan {
 	int wide_grid = 0;           // wide classes: workgroups of every launch
 	uint8_t *d_scratch = nullptr;
 	const void *kernel = nullptr;
+	// all-16 classes: the kernel of each form (null where the form does not
+	// exist) and the tiles of 4 blocks the 16-lane form strides over
+	const void *forms[3] = {nullptr, nullptr, nullptr};
+	int form = FORM_LANE;
+	std::vector<TileJobs> tiles4;
+	int ntiles4 = 0;
+	size_t smem4 = 0;
 	JobDesc *d_jobs = nullptr;   // device addresses (set by plan_pack)
 	TileJobs *d_tiles = nullptr;
+	TileJobs *d_tiles4 = nullptr;
 	double cost = 0;             // relative cost estimate, for ordering
 };
 
 int device_cu_count();
 // Groups jobs into classes (sorted by decreasing cost) with their job and
-// tile descriptors on the host; validates every job.
-int plan_classes(const s2tc_amd_job *jobs, int njobs, std::vector<ClassPlan> &classes);
+// tile descriptors on the host; validates every job.  with_forms: also the
+// descriptors every kernel form of the all-16 classes needs (a batch that
+// may be tuned); otherwise each class has only its default form.
+int plan_classes(const s2tc_amd_job *jobs, int njobs, std::vector<ClassPlan> &classes, bool with_forms = false);
 // Bytes of device memory that hold all descriptors of the classes.
 size_t plan_descriptor_bytes(const std::vector<ClassPlan> &classes);
 // Lays the descriptors out for a device buffer at `dev`: writes them to the

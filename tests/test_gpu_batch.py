@@ -165,8 +165,9 @@ def test_full_batch_matches_reference_digest(gpu):
 @pytest.mark.parametrize("cd", [0, 1, 2, 3, 7, 5])
 def test_single_class_batch(gpu, oracle, cd):
     """A batch whose images all fall into one all-16 class (nrandom 0, one
-    format and refinement) runs that class's two-waves-per-SIMD kernel form;
-    every output equals the oracle, for every format and refinement."""
+    format and refinement) in each of that class's kernel forms (one lane per
+    block with one or two waves per SIMD, 16 lanes per block): every output
+    equals the oracle, for every format and refinement."""
     rng = np.random.default_rng(900 + cd)
     for dxt in range(3):
         for refine in range(3):
@@ -174,16 +175,55 @@ def test_single_class_batch(gpu, oracle, cd):
             for w, h in [(37, 23), (64, 64), (8, 4), (1, 2)]:
                 items.append(dict(rgba=rng.integers(0, 256, w * h * 4).astype(np.uint8), width=w, height=h,
                                   dxt=dxt, cd=cd, nrandom=0, refine=refine, seed=0))
-            _say(f"single-class batch cd={cd} dxt={dxt} refine={refine}")
             imgs = gpu.DeviceImages(items)
             b = gpu.Batch(imgs)
             b.set_schedule(4, 2)
-            b.launch()
-            gpu.synchronize()
-            host = imgs.dst.to_host()
-            for i, it in enumerate(items):
+            exp = []
+            for it in items:
                 q = oracle.rgb565_image(it["rgba"], it["width"], it["height"], 4, {0: 1, 1: 4, 2: 8}[dxt], 0)
-                exp = oracle.encode_quantized(q, it["width"], it["height"], dxt, cd, 0, refine, 0)
-                assert np.array_equal(imgs.output(i, host), exp), (cd, dxt, refine, i)
+                exp.append(oracle.encode_quantized(q, it["width"], it["height"], dxt, cd, 0, refine, 0))
+            forms = [gpu.FORM_LANE, gpu.FORM_ROW16] + ([gpu.FORM_LANE_2WAVES] if cd not in (4, 5, 6) else [])
+            for form in forms:
+                _say(f"single-class batch cd={cd} dxt={dxt} refine={refine} form={form}")
+                b.set_form(0, form)
+                assert b.class_times()[0]["form"] == form
+                imgs.dst.fill(0)
+                b.launch()
+                gpu.synchronize()
+                host = imgs.dst.to_host()
+                for i in range(len(items)):
+                    assert np.array_equal(imgs.output(i, host), exp[i]), (cd, dxt, refine, form, i)
             b.destroy()
             imgs.free()
+
+
+def test_autotuned_forms_mixed_batch(gpu, oracle):
+    """The autotuner may give each all-16 class of a mixed batch a different
+    kernel form; the tuned batch still equals the oracle, and a form a class
+    does not have is refused."""
+    items = small_batch_items(5, 40)
+    rng = np.random.default_rng(8)
+    for (w, h, dxt, cd, nr, rf) in [(33, 17, 2, 7, 0, 2), (20, 12, 0, 3, 0, 1), (64, 32, 1, 4, 0, 1)]:
+        items.append(dict(rgba=rng.integers(0, 256, w * h * 4).astype(np.uint8), width=w, height=h, dxt=dxt,
+                          cd=cd, nrandom=nr, refine=rf, seed=int(rng.integers(0, 2**63))))
+    imgs = gpu.DeviceImages(items)
+    b = gpu.Batch(imgs)
+    _say("autotune a mixed batch")
+    b.autotune(reps=1)
+    ct = b.class_times()
+    _say(f"tuned forms {[(c['normal'], c['cd'], c['form']) for c in ct]}")
+    imgs.dst.fill(0)
+    b.launch()
+    gpu.synchronize()
+    host = imgs.dst.to_host()
+    for i, it in enumerate(items):
+        q = oracle.rgb565_image(it["rgba"], it["width"], it["height"], 4, {0: 1, 1: 4, 2: 8}[it["dxt"]], 0)
+        exp = oracle.encode_quantized(q, it["width"], it["height"], it["dxt"], it["cd"], it["nrandom"],
+                                      it["refine"], it["seed"])
+        assert np.array_equal(imgs.output(i, host), exp), i
+    quick = [k for k, c in enumerate(ct) if c["normal"] == 0]
+    if quick:
+        with pytest.raises(gpu.S2tcError):
+            b.set_form(quick[0], gpu.FORM_ROW16)
+    b.destroy()
+    imgs.free()
