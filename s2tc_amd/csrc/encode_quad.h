@@ -35,8 +35,25 @@ template <> struct QuadW<CD_AVG> { static const bool value = true; enum { R = 4,
 template <> struct QuadW<CD_W0AVG> { static const bool value = true; enum { R = 1, G = 1, B = 1, AG = 1 }; };
 template <> struct QuadW<CD_WAVG> { static const bool value = true; enum { R = 4, G = 4, B = 1, AG = 2 }; };
 
+// Source words: RGBA8 (RAW) or quantized r5 | g6 << 8 | b5 << 16 | a << 24.
+template <bool RAW> struct SrcWord {
+	// DXT1: 0 for a transparent texel (quantized alpha 0), else all ones
+	static S2TC_D uint32_t opaque(uint32_t s) { return RAW ? (uint32_t)((int)s >> 31) : ((s >> 24) ? ~0u : 0u); }
+	static S2TC_D uint32_t to565(uint32_t s)
+	{
+		return RAW ? (((s >> 3) & 0x1fu) << 11) | (((s >> 10) & 0x3fu) << 5) | ((s >> 19) & 0x1fu) : texel565(s);
+	}
+	static S2TC_D int alpha(uint32_t s, int dxt)
+	{
+		const int a = (int)(s >> 24);
+		return RAW ? (dxt == DXT3 ? a >> 4 : a) : a;
+	}
+	// bytes r5, g6, b5 (byte 3 is not read by the users of this word)
+	static S2TC_D uint32_t quantized(uint32_t s) { return RAW ? ((s >> 3) & 0x1f001fu) | ((s >> 2) & 0x3f00u) : s; }
+};
+
 // Per-texel quantities from one source word.
-template <int CD, bool RAW> struct QuadTexel {
+template <int CD, bool RAW> struct QuadTexel : SrcWord<RAW> {
 	typedef QuadW<CD> W;
 	uint32_t u;  // bytes (r, AG*g, b, 0)
 	int d;       // |t|_w^2 = d(t, black)
@@ -56,17 +73,6 @@ template <int CD, bool RAW> struct QuadTexel {
 		else if (CD == CD_W0AVG) v = u;          // (r, g, b)
 		else v = u + 3u * (rb & 0x1fu);          // (4r, 2g, b) against (r, 2g, b)
 		d = (int)__builtin_amdgcn_udot4(u, v, 0u, false);
-	}
-	// DXT1: 0 for a transparent texel (quantized alpha 0), else all ones
-	static S2TC_D uint32_t opaque(uint32_t s) { return RAW ? (uint32_t)((int)s >> 31) : ((s >> 24) ? ~0u : 0u); }
-	static S2TC_D uint32_t to565(uint32_t s)
-	{
-		return RAW ? (((s >> 3) & 0x1fu) << 11) | (((s >> 10) & 0x3fu) << 5) | ((s >> 19) & 0x1fu) : texel565(s);
-	}
-	static S2TC_D int alpha(uint32_t s, int dxt)
-	{
-		const int a = (int)(s >> 24);
-		return RAW ? (dxt == DXT3 ? a >> 4 : a) : a;
 	}
 };
 
@@ -125,26 +131,31 @@ S2TC_D QSums plane_sums(const QuadPlanes &P, const uint32_t (&m)[4])
 }
 
 // s2tc_evaluate_colors_result_t<color_t, bigcolor_t, 1>::evaluate (:542-551)
-template <int CD>
-S2TC_D bool quad_eval(int n0, const QSums &s0, int n1, const QSums &s1, uint32_t &c0, uint32_t &c1)
+// from sums of the dot bytes (r, AG*g, b)
+template <int AG>
+S2TC_D bool quad_eval_ag(int n0, const QSums &s0, int n1, const QSums &s1, uint32_t &c0, uint32_t &c1)
 {
-	typedef QuadW<CD> W;
 	if (!n0 && !n1) return false;
 	if (n0) {
 		const Recip q((uint32_t)(n0 << 1));
 		const int r = (int)q.div((uint32_t)((s0.r << 1) + n0)) & 31;
-		const int g = (int)q.div((uint32_t)(((s0.g / W::AG) << 1) + n0)) & 63;
+		const int g = (int)q.div((uint32_t)(((s0.g / AG) << 1) + n0)) & 63;
 		const int b = (int)q.div((uint32_t)((s0.b << 1) + n0)) & 31;
 		c0 = (uint32_t)((r << 11) | (g << 5) | b);
 	}
 	if (n1) {
 		const Recip q((uint32_t)(n1 << 1));
 		const int r = (int)q.div((uint32_t)((s1.r << 1) + n1)) & 31;
-		const int g = (int)q.div((uint32_t)(((s1.g / W::AG) << 1) + n1)) & 63;
+		const int g = (int)q.div((uint32_t)(((s1.g / AG) << 1) + n1)) & 63;
 		const int b = (int)q.div((uint32_t)((s1.b << 1) + n1)) & 31;
 		c1 = (uint32_t)((r << 11) | (g << 5) | b);
 	}
 	return true;
+}
+template <int CD>
+S2TC_D bool quad_eval(int n0, const QSums &s0, int n1, const QSums &s1, uint32_t &c0, uint32_t &c1)
+{
+	return quad_eval_ag<QuadW<CD>::AG>(n0, s0, n1, s1, c0, c1);
 }
 
 // One s2tc_try_encode_block pass for endpoints c0, c1 (565 words).  Returns

@@ -6,6 +6,7 @@
 // parallel.
 #include "encode_all16.h"
 #include "encode_normal.h"
+#include "encode_lin.h"
 #include "encode_quad.h"
 #include "kernel_table.h"
 
@@ -99,8 +100,16 @@ __global__ __launch_bounds__(256, 4) void k_fast(const uint32_t *__restrict__ sr
 		const uint32_t *blk = src + (size_t)(4 * by) * (size_t)width + 4 * bx;
 		o = encode_block_quad_any<CD, DXT, RAW>(tx, blk, width, w, h, refine);
 	} else {
-		load_block<DXT, RAW>(src, width, bx, by, w, h, vec != 0, tx);
-		o = encode_block_fast<CD, DXT>(tx, w, h, refine);
+		bool lin = false;
+		if constexpr (LinW<CD>::value) lin = refine != REFINE_LOOP;
+		if (lin) {
+			load_block<DXT, false>(src, width, bx, by, w, h, vec != 0, tx);
+			const uint32_t *blk = src + (size_t)(4 * by) * (size_t)width + 4 * bx;
+			if constexpr (LinW<CD>::value) o = encode_block_lin_any<CD, DXT, RAW>(tx, blk, width, w, h, refine);
+		} else {
+			load_block<DXT, RAW>(src, width, bx, by, w, h, vec != 0, tx);
+			o = encode_block_fast<CD, DXT>(tx, w, h, refine);
+		}
 	}
 	store_block<DXT>(dst, b, o);
 }
@@ -136,6 +145,14 @@ __global__ __launch_bounds__(256) void k_fast_interior(const uint32_t *__restric
 		} else {
 			opaque_to_compiler(tx);
 			o = encode_block_quad<CD, DXT, true, RAW, false>(tx, blk, width, 4, 4, refine);
+		}
+	} else if constexpr (LinW<CD>::value && REFINE != REFINE_LOOP) {
+		if (DXT == DXT1 && __all((int)(RAW ? tx_min16(tx) >> 31 : tx_min16(tx) >> 24) != 0)) {
+			opaque_to_compiler(tx);
+			o = encode_block_lin<CD, DXT, true, RAW, true>(tx, blk, width, 4, 4, refine);
+		} else {
+			opaque_to_compiler(tx);
+			o = encode_block_lin<CD, DXT, true, RAW, false>(tx, blk, width, 4, 4, refine);
 		}
 	} else {
 		if (RAW) {
@@ -274,6 +291,11 @@ __global__ __launch_bounds__(256) void k_fast_batch(const JobDesc *__restrict__ 
 			const uint32_t *blk = j.src + (size_t)(4 * by) * (size_t)j.width + 4 * bx;
 			if (j.raw) o = encode_block_quad_any<CD, DXT, true>(tx, blk, j.width, w, h, REFINE);
 			else o = encode_block_quad_any<CD, DXT, false>(tx, blk, j.width, w, h, REFINE);
+		} else if constexpr (LinW<CD>::value && REFINE != REFINE_LOOP) {
+			load_block<DXT, false>(j.src, j.width, bx, by, w, h, vec, tx);
+			const uint32_t *blk = j.src + (size_t)(4 * by) * (size_t)j.width + 4 * bx;
+			if (j.raw) o = encode_block_lin_any<CD, DXT, true>(tx, blk, j.width, w, h, REFINE);
+			else o = encode_block_lin_any<CD, DXT, false>(tx, blk, j.width, w, h, REFINE);
 		} else {
 			if (j.raw) load_block<DXT, true>(j.src, j.width, bx, by, w, h, vec, tx);
 			else load_block<DXT, false>(j.src, j.width, bx, by, w, h, vec, tx);

@@ -46,6 +46,10 @@ S2TC_D int wadd(int a, int b) { return (int)((uint32_t)a + (uint32_t)b); }
 // the low 32 bits of a * b (+ c), i.e. the int32 wrap of the reference's
 // expressions.  Written out because the compiler otherwise picks the
 // quarter-rate 32-bit multiplies (v_mul_lo_u32, v_mad_u64_u32) for them.
+// Never feed them a v_dot4 result: gfx950 needs three wait states before a
+// VALU instruction reads a dot product's result, and the compiler inserts
+// them only in front of instructions it generates, not in front of asm (a
+// multiply one or two instructions after the dot reads the old register).
 S2TC_D int mul_i24(int a, int b)
 {
 	int r;

@@ -41,9 +41,61 @@ static bool fs_mulhi_division_exact()
 	return true;
 }
 
+// SHRR of s2tc_algorithm.cpp:215 (no wrap in the ranges below)
+static int shrr(int a, int n) { return (a + (1 << (n - 1))) >> n; }
+
+// The quick encoder's linear endpoint test for RGB, YUV and SRGB_MIXED rests
+// on these identities, checked for every component difference (RGB, YUV) or
+// every term value (SRGB_MIXED) the distances can see:
+//   RGB (:256-267):  4 d = 32 y'^2 + 2 u'^2 + v'^2 + 2 (u' & 1) - (v' & 1),
+//     y' = 21 dr + 36 dg + 7 db, u' = 80 dr - 36 dg - 7 db,
+//     v' = -21 dr - 36 dg + 94 db (the reference's y, u, v halved), with
+//     u' & 1 = db & 1 and v' & 1 = dr & 1;
+//   YUV (:243-254):  16 d = 32 y^2 + 2 u^2 + v^2 + cu[u & 3] + cv[v & 7];
+//   SRGB_MIXED (:298-315): 4 SHRR(u^2, 1) = 2 u^2 + 2 (u & 1) and
+//     4 SHRR(v^2, 2) = v^2 - (v & 1).
+static const int kCu[4] = {0, -2, 8, -2};
+static const int kCv[8] = {0, -1, -4, 7, 0, 7, -4, -1};
+
+static bool linear_test_identities()
+{
+	for (int dr = -31; dr <= 31; ++dr)
+		for (int dg = -63; dg <= 63; ++dg)
+			for (int db = -31; db <= 31; ++db) {
+				{
+					const int y = dr * 42 + dg * 72 + db * 14, u = dr * 202 - y, v = db * 202 - y;
+					const long long d = ((long long)(y * y) << 1) + shrr(u * u, 3) + shrr(v * v, 4);
+					const int y2 = 21 * dr + 36 * dg + 7 * db, u2 = 80 * dr - 36 * dg - 7 * db,
+					          v2 = -21 * dr - 36 * dg + 94 * db;
+					const long long q = 32LL * y2 * y2 + 2LL * u2 * u2 + (long long)v2 * v2 + 2 * (u2 & 1) - (v2 & 1);
+					if (4 * d != q || (u2 & 1) != (db & 1) || (v2 & 1) != (dr & 1)) {
+						printf("rgb identity: %d %d %d\n", dr, dg, db);
+						return false;
+					}
+				}
+				{
+					const int y = dr * 60 + dg * 59 + db * 22, u = dr * 202 - y, v = db * 202 - y;
+					const long long d = ((long long)(y * y) << 1) + shrr(u * u, 3) + shrr(v * v, 4);
+					const long long q = 32LL * y * y + 2LL * u * u + (long long)v * v + kCu[u & 3] + kCv[v & 7];
+					if (16 * d != q) {
+						printf("yuv identity: %d %d %d\n", dr, dg, db);
+						return false;
+					}
+				}
+			}
+	for (int u = -20000; u <= 20000; ++u) {
+		if (4LL * shrr(u * u, 1) != 2LL * u * u + 2 * (u & 1) || 4LL * shrr(u * u, 2) != (long long)u * u - (u & 1)) {
+			printf("srgb_mixed identity: %d\n", u);
+			return false;
+		}
+	}
+	return true;
+}
+
 int main()
 {
 	if (!fs_mulhi_division_exact()) return 1;
+	if (!linear_test_identities()) return 1;
 	std::vector<size_t> lens;
 	for (size_t n = 0; n < 2048; ++n) lens.push_back(n);
 	// chunk tails of large uploads: multiples of 64 * nt plus a few bytes,
