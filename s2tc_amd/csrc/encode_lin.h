@@ -75,22 +75,22 @@ template <> struct LinDist<CD_RGB> {
 	{
 		return (mul_i24(f.ny, f.ny) << 3) + (mad_i24(f.u, f.u, 1) >> 1) + (mad_i24(f.v, f.v, 2) >> 2);
 	}
-	// u' & 1 = b & 1, v' & 1 = r & 1
-	static S2TC_D int pu(uint32_t q, const LinF &) { return (int)((q >> 16) & 1u); }
-	static S2TC_D int pv(uint32_t q, const LinF &) { return (int)(q & 1u); }
 };
 
 // YUV: y = 60 r + 59 g + 22 b, u = 202 r - y, v = 202 b - y.
 template <> struct LinDist<CD_YUV> {
 	static const bool PARITY = false;
-	typedef int Extra;
-	static S2TC_D LinF feat(uint32_t q, Extra)
+	typedef int Extra;  // -y of the texel, kept from the search for the pass
+	static S2TC_D Extra extra(uint32_t q)
 	{
 		const int r = (int)(q & 0x1fu), g = (int)((q >> 8) & 0x3fu), b = (int)((q >> 16) & 0x1fu);
-		const int ny = mad_i24(r, -60, mad_i24(g, -59, mul_i24(b, -22)));
+		return mad_i24(r, -60, mad_i24(g, -59, mul_i24(b, -22)));
+	}
+	static S2TC_D LinF feat(uint32_t q, Extra ny)
+	{
+		const int r = (int)(q & 0x1fu), b = (int)((q >> 16) & 0x1fu);
 		return LinF{ny, mad_i24(r, 202, ny), mad_i24(b, 202, ny)};
 	}
-	static S2TC_D Extra extra(uint32_t) { return 0; }
 	static S2TC_D LinF feat565(uint32_t c)
 	{
 		const int r = c_r(c), g = c_g(c), b = c_b(c);
@@ -102,8 +102,6 @@ template <> struct LinDist<CD_YUV> {
 	{
 		return (mul_i24(f.ny, f.ny) << 1) + (mad_i24(f.u, f.u, 4) >> 3) + (mad_i24(f.v, f.v, 8) >> 4);
 	}
-	static S2TC_D int pu(uint32_t, const LinF &) { return 0; }
-	static S2TC_D int pv(uint32_t, const LinF &) { return 0; }
 };
 
 // SRGB_MIXED: Y = srgb_get_y of the colour (table), u = 191 r - Y, v = 191 b - Y.
@@ -130,8 +128,6 @@ template <> struct LinDist<CD_SRGB_MIXED> {
 	{
 		return (mul_i24(f.ny, f.ny) << 3) + (mad_i24(f.u, f.u, 1) >> 1) + (mad_i24(f.v, f.v, 2) >> 2);
 	}
-	static S2TC_D int pu(uint32_t, const LinF &f) { return f.u & 1; }
-	static S2TC_D int pv(uint32_t, const LinF &f) { return f.v & 1; }
 };
 
 // Q(F) = 32 ny^2 + 2 u^2 + v^2 of an endpoint (below 2^31 for every distance here)
@@ -158,15 +154,25 @@ S2TC_D void rot8(uint32_t hi, uint32_t lo, int k, uint32_t &rhi, uint32_t &rlo)
 #define S2TC_YUV_CV_LO 0x0f040708u
 #define S2TC_YUV_CV_HI 0x07040f08u
 
-// The per-pass part of the test for endpoints e0, e1:
-//   t(x) = K + 64 ny_x dny + 4 u_x du + 2 v_x dv + correction(x) = s (d1 - d0)
-// Parity form: correction = (2 su - sv) + pu (-4 su) + pv (2 sv), with
-// s? = (endpoint 1 parity) - (endpoint 0 parity); the constant joins K, the
-// rest is two multiply-adds of the texel's parity bits (pu, pv).
+// The per-pass part of the test for endpoints e0, e1.  With
+// dny = 64 (ny0 - ny1), du = 4 (u0 - u1), dv = 2 (v0 - v1) and K = Q(F1) - Q(F0),
+//   s (d1 - d0) = K + ny_x dny + u_x du + v_x dv + correction(x),
+// and u_x, v_x are the texel's channels plus ny_x:
+//   RGB         u = 101 r + ny, v = 101 b + ny, ny = -(21 r + 36 g + 7 b), so the
+//               linear part is K + r wr + g wg + b wb in the channels;
+//   YUV         u = 202 r + ny, v = 202 b + ny: K + ny_x A + r wr + b wb, A = dny + du + dv;
+//   SRGB_MIXED  u = 191 r + ny, v = 191 b + ny: likewise.
+// Every weight lies in (-2^23, 2^23) for every pair of endpoints (checked on
+// the host), as the 24-bit multiply-adds need.
+// Parity form (RGB, SRGB_MIXED): correction = (2 su - sv) + pu (-4 su) + pv (2 sv)
+// with s? = (endpoint 1 parity) - (endpoint 0 parity) and pu = u_x & 1,
+// pv = v_x & 1 (RGB: b & 1 and r & 1; SRGB_MIXED: (r ^ ny) & 1, (b ^ ny) & 1);
+// the constant joins K.
 // YUV: correction = tu[u_x & 3] + tv[v_x & 7] - 32 from per-pass byte tables
-// tu = cu[(j - u1) & 3] - cu[(j - u0) & 3] + 16, likewise tv over 8 entries.
+// tu = cu[(j - u1) & 3] - cu[(j - u0) & 3] + 16, likewise tv over 8 entries;
+// u_x & 3 = (2 r + g + 2 b) & 3 and v_x & 7 = (4 r + 5 g + 4 b) & 7.
 struct LinPass {
-	int K, dny, du, dv;
+	int K, A, wr, wg, wb;
 	int wu, wv;               // parity form
 	uint32_t tu, tvlo, tvhi;  // YUV
 };
@@ -178,9 +184,17 @@ S2TC_D LinPass lin_pass_setup(uint32_t c0, uint32_t c1)
 	const LinF e0 = L::feat565(c0), e1 = L::feat565(c1);
 	LinPass p;
 	p.K = lin_q(e1) - lin_q(e0);
-	p.dny = 64 * (e0.ny - e1.ny);
-	p.du = 4 * (e0.u - e1.u);
-	p.dv = 2 * (e0.v - e1.v);
+	const int dny = 64 * (e0.ny - e1.ny), du = 4 * (e0.u - e1.u), dv = 2 * (e0.v - e1.v);
+	p.A = dny + du + dv;
+	const int k = CD == CD_RGB ? 101 : CD == CD_YUV ? 202 : 191;
+	p.wr = k * du;
+	p.wb = k * dv;
+	p.wg = 0;
+	if (CD == CD_RGB) {
+		p.wr -= 21 * p.A;
+		p.wg = -36 * p.A;
+		p.wb -= 7 * p.A;
+	}
 	if constexpr (L::PARITY) {
 		const int su = (e1.u & 1) - (e0.u & 1), sv = (e1.v & 1) - (e0.v & 1);
 		p.K += 2 * su - sv;
@@ -200,19 +214,26 @@ S2TC_D LinPass lin_pass_setup(uint32_t c0, uint32_t c1)
 	return p;
 }
 
-// s (d1 - d0) for the texel word q (bytes r5, g6, b5) with its kept extra
+// s (d1 - d0) for the texel word q (bytes r5, g6, b5) with its kept extra;
+// the products wrap in int32 where they exceed it, the sum does not
 template <int CD>
 S2TC_D int lin_test(const LinPass &p, uint32_t q, typename LinDist<CD>::Extra ex)
 {
-	typedef LinDist<CD> L;
-	const LinF f = L::feat(q, ex);
-	// the products wrap in int32 where they exceed it; the sum does not
-	int t = mad_i24(f.ny, p.dny, mad_i24(f.u, p.du, mad_i24(f.v, p.dv, p.K)));
-	if constexpr (L::PARITY) {
-		t = mad_i24(L::pu(q, f), p.wu, mad_i24(L::pv(q, f), p.wv, t));
+	const int r = (int)(q & 0x1fu), b = (int)((q >> 16) & 0x1fu);
+	int t;
+	if constexpr (CD == CD_RGB) {
+		const int g = (int)((q >> 8) & 0x3fu);
+		t = mad_i24(r, p.wr, mad_i24(g, p.wg, mad_i24(b, p.wb, p.K)));
+		t = mad_i24(b & 1, p.wu, mad_i24(r & 1, p.wv, t));
+	} else if constexpr (CD == CD_SRGB_MIXED) {
+		t = mad_i24(ex, p.A, mad_i24(r, p.wr, mad_i24(b, p.wb, p.K)));
+		t = mad_i24((r ^ ex) & 1, p.wu, mad_i24((b ^ ex) & 1, p.wv, t));
 	} else {
-		const uint32_t tu = __builtin_amdgcn_perm(p.tu, p.tu, ((uint32_t)f.u & 3u) | 0x0c0c0c00u);
-		const uint32_t tv = __builtin_amdgcn_perm(p.tvhi, p.tvlo, ((uint32_t)f.v & 7u) | 0x0c0c0c00u);
+		t = mad_i24(ex, p.A, mad_i24(r, p.wr, mad_i24(b, p.wb, p.K)));
+		const uint32_t iu = __builtin_amdgcn_udot4(q & 0x00ffffffu, 0x00020102u, 0u, false);
+		const uint32_t iv = __builtin_amdgcn_udot4(q & 0x00ffffffu, 0x00040504u, 0u, false);
+		const uint32_t tu = __builtin_amdgcn_perm(p.tu, p.tu, (iu & 3u) | 0x0c0c0c00u);
+		const uint32_t tv = __builtin_amdgcn_perm(p.tvhi, p.tvlo, (iv & 7u) | 0x0c0c0c00u);
 		t = (int)((uint32_t)t + tu + tv);
 	}
 	return t;

@@ -1,6 +1,7 @@
 // Host-only checks of library helpers, compiled with g++ against the
 // library's own headers by tests/test_host_checks.py.  Prints "ok" and exits
 // 0 when every check passes; otherwise prints the failing case and exits 1.
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
@@ -92,10 +93,56 @@ static bool linear_test_identities()
 	return true;
 }
 
+// The per-pass weights of the linear test are factors of 24-bit
+// multiply-adds: every one must lie in (-2^23, 2^23) for every endpoint pair.
+// They depend on the endpoints' feature differences only (dny = 64 (ny0 -
+// ny1), du = 4 (u0 - u1), dv = 2 (v0 - v1), A = dny + du + dv):
+//   RGB         wr = 101 du - 21 A, wg = -36 A, wb = 101 dv - 7 A
+//   YUV         A, wr = 202 du, wb = 202 dv
+//   SRGB_MIXED  A, wr = 191 du, wb = 191 dv (dY over every luminance difference)
+static bool fits24(long long w) { return w > -(1LL << 23) && w < (1LL << 23); }
+
+static bool linear_test_weight_ranges()
+{
+	for (int dr = -31; dr <= 31; ++dr)
+		for (int dg = -63; dg <= 63; ++dg)
+			for (int db = -31; db <= 31; ++db) {
+				{
+					const long long y = 21 * dr + 36 * dg + 7 * db, u = 101LL * dr - y, v = 101LL * db - y;
+					const long long A = 64 * -y + 4 * u + 2 * v;
+					if (!fits24(A) || !fits24(101 * 4 * u - 21 * A) || !fits24(-36 * A) || !fits24(101 * 2 * v - 7 * A)) {
+						printf("rgb weights: %d %d %d\n", dr, dg, db);
+						return false;
+					}
+				}
+				{
+					const long long y = 60 * dr + 59 * dg + 22 * db, u = 202LL * dr - y, v = 202LL * db - y;
+					const long long A = 64 * -y + 4 * u + 2 * v;
+					if (!fits24(A) || !fits24(202 * 4 * u) || !fits24(202 * 2 * v)) {
+						printf("yuv weights: %d %d %d\n", dr, dg, db);
+						return false;
+					}
+				}
+			}
+	// SRGB_MIXED: Y = (int)(sqrtf(37 (84 r^2 + 72 g^2 + 28 b^2)) + 0.5f) is at most that of (31, 63, 31)
+	const int ymax = (int)(sqrtf((float)(37 * (84 * 961 + 72 * 3969 + 28 * 961))) + 0.5f);
+	for (int dr = -31; dr <= 31; ++dr)
+		for (int db = -31; db <= 31; ++db)
+			for (int dy = -ymax; dy <= ymax; ++dy) {
+				const long long u = 191LL * dr - dy, v = 191LL * db - dy, A = 64LL * -dy + 4 * u + 2 * v;
+				if (!fits24(A) || !fits24(191 * 4 * u) || !fits24(191 * 2 * v)) {
+					printf("srgb_mixed weights: %d %d %d\n", dr, db, dy);
+					return false;
+				}
+			}
+	return true;
+}
+
 int main()
 {
 	if (!fs_mulhi_division_exact()) return 1;
 	if (!linear_test_identities()) return 1;
+	if (!linear_test_weight_ranges()) return 1;
 	std::vector<size_t> lens;
 	for (size_t n = 0; n < 2048; ++n) lens.push_back(n);
 	// chunk tails of large uploads: multiples of 64 * nt plus a few bytes,
