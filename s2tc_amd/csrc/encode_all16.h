@@ -123,46 +123,61 @@ S2TC_D BlockOut encode_block_all16(const uint32_t (&tx)[16], int w, int h, int r
 			a1 = 0;
 		}
 	} else {
-		int bi, bj;
-		{
-			// the distances of the upper triangle; texels outside the
-			// candidate set contribute nothing to any sum
-			int D[120];
-			{
-				// every distance is computed (no per-pair branch) and then
-				// cleared where a slot is not a candidate
-				F f[16];
-				uint32_t um[16];
-#pragma unroll
-				for (int k = 0; k < 16; ++k) {
-					f[k] = Dd::featq(tx[k]);
-					um[k] = 0u - ((use >> k) & 1u);
-				}
-#pragma unroll
-				for (int i = 0; i < 16; ++i)
-#pragma unroll
-					for (int j = i + 1; j < 16; ++j)
-						D[tri(i, j)] = (int)((uint32_t)Dd::d(f[i], f[j]) & um[i] & um[j]);
-			}
-			__builtin_amdgcn_sched_barrier(0);
-			uint32_t p;
-			if constexpr (DistBound<CD>::value != 0) {
-				const int none[16] = {};
-				p = fold_pairs_key<false>(D, none, use);
-			} else {
-				p = fold_pairs<IsSymmetric<CD>::value>(D, use);
-			}
-			bi = (int)(p >> 4);
-			bj = (int)(p & 15u);
-		}
-		uint32_t v0 = 0, v1 = 0;
+		// Every candidate of one colour: whichever pair the scan picks, both of
+		// its colours are that one (:996-1001 takes c[0], c[1] of the pair), so
+		// the scan is skipped; a wave of such blocks skips it altogether.
+		uint32_t lo = 0xffffffffu, hi = 0u;
 #pragma unroll
 		for (int k = 0; k < 16; ++k) {
-			v0 = k == bi ? texel565(tx[k]) : v0;
-			v1 = k == bj ? texel565(tx[k]) : v1;
+			const uint32_t cw = tx[k] & 0xffffffu;
+			const bool u = (use >> k) & 1u;
+			lo = u && cw < lo ? cw : lo;
+			hi = u && cw > hi ? cw : hi;
 		}
-		c0 = v0;
-		c1 = v1;
+		if (lo == hi) {
+			c0 = c1 = texel565(lo);
+		} else {
+			int bi, bj;
+			{
+				// the distances of the upper triangle; texels outside the
+				// candidate set contribute nothing to any sum
+				int D[120];
+				{
+					// every distance is computed (no per-pair branch) and then
+					// cleared where a slot is not a candidate
+					F f[16];
+					uint32_t um[16];
+#pragma unroll
+					for (int k = 0; k < 16; ++k) {
+						f[k] = Dd::featq(tx[k]);
+						um[k] = 0u - ((use >> k) & 1u);
+					}
+#pragma unroll
+					for (int i = 0; i < 16; ++i)
+#pragma unroll
+						for (int j = i + 1; j < 16; ++j)
+							D[tri(i, j)] = (int)((uint32_t)Dd::d(f[i], f[j]) & um[i] & um[j]);
+				}
+				__builtin_amdgcn_sched_barrier(0);
+				uint32_t p;
+				if constexpr (DistBound<CD>::value != 0) {
+					const int none[16] = {};
+					p = fold_pairs_key<false>(D, none, use);
+				} else {
+					p = fold_pairs<IsSymmetric<CD>::value>(D, use);
+				}
+				bi = (int)(p >> 4);
+				bj = (int)(p & 15u);
+			}
+			uint32_t v0 = 0, v1 = 0;
+#pragma unroll
+			for (int k = 0; k < 16; ++k) {
+				v0 = k == bi ? texel565(tx[k]) : v0;
+				v1 = k == bj ? texel565(tx[k]) : v1;
+			}
+			c0 = v0;
+			c1 = v1;
+		}
 		if (DXT == DXT5 && !auni) {
 			// keep the alpha triangle out of the colour triangle's live range
 			__builtin_amdgcn_sched_barrier(0);

@@ -462,6 +462,19 @@ S2TC_D void encode_block_normal(uint32_t tex, int t, int w, int h, int dxt, int 
 	const bool none = n == 0;
 	if (none) n = 1;
 	if (none && t == 0) sm.cand[0] = 0u;
+	// One colour among the candidates (the random ones are drawn inside the
+	// block colours' box, :962-993, so they share it): every pair the scan can
+	// pick has that colour, and the scan is skipped.  The same for DXT5 alpha,
+	// except where the last-miplevel hack pairs a lone texel with alpha 0.
+	const int vmin = row_min(use ? (int)v : 0xffff), vmax = row_max(use ? (int)v : 0);
+	const bool cuni = none || vmin == vmax;
+	bool auni = false;
+	int aone = 0;
+	if (dxt == DXT5) {
+		const int amin = row_min(use ? alpha : 255), amax = row_max(use ? alpha : 0);
+		auni = amin == amax && (nrandom > 0 || n >= 2);
+		aone = amin;
+	}
 	__syncthreads();
 	int m = n;
 	if (nrandom > 0) {  // (:962-993)
@@ -493,7 +506,7 @@ S2TC_D void encode_block_normal(uint32_t tex, int t, int w, int h, int dxt, int 
 	__syncthreads();
 
 	// dists[i][k] rows (:373-392): square part dist(c[min], c[max]), box part dist(c[i], c[k])
-	for (int p = 0; p < pbw; ++p) {
+	for (int p = 0; p < (cuni ? 0 : pbw); ++p) {
 		const int i = 16 * p + t;
 		const bool irow = i < m;
 		const F fi = D::feat(sm.cand[irow ? i : 0] & 0xffffu);
@@ -511,18 +524,24 @@ S2TC_D void encode_block_normal(uint32_t tex, int t, int w, int h, int dxt, int 
 		for (int k = 0; k < 4; ++k) rp[k] = make_int4(dv[4 * k], dv[4 * k + 1], dv[4 * k + 2], dv[4 * k + 3]);
 	}
 	__syncthreads();
-	uint32_t cbest;
-	if constexpr (DistBound<CD>::value != 0) cbest = reduce_rows_key<false>(sm.rows, t, m, pbw);
-	else if constexpr (IsSymmetric<CD>::value) cbest = reduce_rows_key<true>(sm.rows, t, m, pbw);
-	else cbest = reduce_rows<true>(sm.rows, t, m, pbw);
-	const int ci = min((int)(cbest >> 16), m - 1), cj = min((int)(cbest & 0xffffu), m - 1);
-	uint32_t c0 = sm.cand[ci] & 0xffffu, c1 = sm.cand[cj] & 0xffffu;
+	uint32_t c0, c1;
+	if (cuni) {
+		c0 = c1 = none ? 0u : (uint32_t)vmin;
+	} else {
+		uint32_t cbest;
+		if constexpr (DistBound<CD>::value != 0) cbest = reduce_rows_key<false>(sm.rows, t, m, pbw);
+		else if constexpr (IsSymmetric<CD>::value) cbest = reduce_rows_key<true>(sm.rows, t, m, pbw);
+		else cbest = reduce_rows<true>(sm.rows, t, m, pbw);
+		const int ci = min((int)(cbest >> 16), m - 1), cj = min((int)(cbest & 0xffffu), m - 1);
+		c0 = sm.cand[ci] & 0xffffu;
+		c1 = sm.cand[cj] & 0xffffu;
+	}
 	int a0 = 0, a1 = 0;
 	// reduce_colors_inplace_2fixpoints for DXT5 alpha (:1006, :415-478).  The
 	// barriers are outside the format test: the four blocks of a workgroup
 	// may have different formats in the batch kernels.
 	__syncthreads();
-	if (dxt == DXT5) {
+	if (dxt == DXT5 && !auni) {
 		for (int p = 0; p < pbw; ++p) {
 			const int i = 16 * p + t;
 			const bool irow = i < m;
@@ -542,7 +561,9 @@ S2TC_D void encode_block_normal(uint32_t tex, int t, int w, int h, int dxt, int 
 		}
 	}
 	__syncthreads();
-	if (dxt == DXT5) {
+	if (dxt == DXT5 && auni) {
+		a0 = a1 = aone;
+	} else if (dxt == DXT5) {
 		const uint32_t abest = reduce_rows_key<false>(sm.rows, t, m, pbw);
 		const int ai = min((int)(abest >> 16), m - 1), aj = min((int)(abest & 0xffffu), m - 1);
 		a0 = (int)(sm.cand[ai] >> 16);

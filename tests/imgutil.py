@@ -34,6 +34,45 @@ def random_quantized(rng, w, h, dxt, kind="random"):
     return q.reshape(-1)
 
 
+def flat_blocks(rng, w, h, dxt):
+    """Quantized texels whose blocks are mostly of one colour among the texels
+    that take part in the search: the top half of the block rows has only
+    such blocks, the bottom half mixes them with the exceptions (one texel
+    off, a lone opaque texel, random blocks)."""
+    ab = (1 << ALPHABITS[dxt]) - 1
+    bw, bh = (w + 3) // 4, (h + 3) // 4
+    q = np.zeros((bh * 4, bw * 4, 4), np.uint8)
+    for by in range(bh):
+        for bx in range(bw):
+            b = q[4 * by:4 * by + 4, 4 * bx:4 * bx + 4]
+            col = (rng.integers(0, 32), rng.integers(0, 64), rng.integers(0, 32))
+            kind = rng.integers(0, 4) if by < bh // 2 else rng.integers(0, 8)
+            b[..., 0], b[..., 1], b[..., 2] = col
+            b[..., 3] = ab
+            if kind == 1:    # one colour, alpha varies
+                b[..., 3] = rng.integers(0, ab + 1, (4, 4))
+            elif kind == 2:  # one colour; transparent texels of other colours
+                t = rng.random((4, 4)) < 0.4
+                b[..., 3] = np.where(t, 0, ab)
+                b[..., 0] = np.where(t, rng.integers(0, 32, (4, 4)), col[0])
+            elif kind == 3:  # one colour, one alpha below the maximum
+                b[..., 3] = rng.integers(0, ab + 1)
+            elif kind == 4:  # one texel of another colour
+                b[rng.integers(0, 4), rng.integers(0, 4), 1] ^= 1
+            elif kind == 5:  # a lone opaque texel
+                b[..., 3] = 0
+                b[..., 0] = rng.integers(0, 32, (4, 4))
+                b[rng.integers(0, 4), rng.integers(0, 4), 3] = ab
+            elif kind == 6:  # everything transparent
+                b[..., 3] = 0
+            elif kind == 7:  # random block
+                b[..., 0] = rng.integers(0, 32, (4, 4))
+                b[..., 1] = rng.integers(0, 64, (4, 4))
+                b[..., 2] = rng.integers(0, 32, (4, 4))
+                b[..., 3] = rng.integers(0, ab + 1, (4, 4))
+    return np.ascontiguousarray(q[:h, :w]).reshape(-1)
+
+
 def special_blocks(dxt):
     """A 4-block-row image of hand-picked 4x4 blocks (returns q, w, h)."""
     ab = (1 << ALPHABITS[dxt]) - 1

@@ -63,6 +63,18 @@ def test_special_blocks(gpu, oracle, dxt):
         _compare(gpu, oracle, q, w, h, dxt, cd, nr, refine, 11, "special")
 
 
+@pytest.mark.parametrize("cd", range(8))
+def test_one_colour_blocks(gpu, oracle, cd):
+    """Blocks with one colour (and alpha) among their candidates skip the pair
+    scans: whole waves and workgroups of them (top half), mixed with other
+    blocks (bottom half), partial blocks at the right and bottom edges."""
+    rng = np.random.default_rng(300 + cd)
+    for (w, h), dxt in itertools.product([(256, 64), (37, 23)], range(3)):
+        q = imgutil.flat_blocks(rng, w, h, dxt)
+        for nr, refine in itertools.product([-1, 0, 5, 64], range(3)):
+            _compare(gpu, oracle, q, w, h, dxt, cd, nr, refine, 5 + cd, "flat")
+
+
 def test_raw_rgba8_source(gpu, oracle):
     """Fused DITHER_NONE quantization on load equals rgb565_image + encode."""
     rng = np.random.default_rng(5)
