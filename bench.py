@@ -46,16 +46,15 @@ def measured_traffic():
     return out
 
 
-def image_items(name):
+def image_items(s, name):
+    """One single-image workload, quantized with its own dither (SIMPLE for
+    all three, as tx_compress_dxtn does by default) before anything is timed;
+    the items carry the quantized texels (SRC_QUANTIZED)."""
     from s2tc_amd import workloads
     wl = workloads.IMAGE_WORKLOADS[name]
     src = workloads.make_image_source(wl)
-    if wl["srccomps"] == 3:
-        import numpy as np
-        rgba = np.full(wl["width"] * wl["height"] * 4, 255, np.uint8)
-        rgba.reshape(-1, 4)[:, :3] = src.reshape(-1, 3)
-        src = rgba
-    return [dict(rgba=src, width=wl["width"], height=wl["height"], dxt=wl["dxt"], cd=wl["cd"],
+    q = s.rgb565_image(src, wl["width"], wl["height"], wl["srccomps"], {0: 1, 1: 4, 2: 8}[wl["dxt"]], wl["dither"])
+    return [dict(rgba=q, width=wl["width"], height=wl["height"], dxt=wl["dxt"], cd=wl["cd"],
                  nrandom=wl["nrandom"], refine=wl["refine"], seed=wl["seed"])]
 
 
@@ -618,8 +617,8 @@ def main():
         del textures
         items = imgs.items
     else:
-        items = image_items(a.workload)
-        imgs = s.DeviceImages(items)
+        items = image_items(s, a.workload)
+        imgs = s.DeviceImages(items, src_kind=s.SRC_QUANTIZED)
     log(f"workload {a.workload}: {len(items)} images on the device after {time.time() - t0:.1f} s")
     texels = imgs.texels
     batch = s.Batch(imgs)

@@ -223,7 +223,7 @@ __global__ __launch_bounds__(64) void k_normal(const uint32_t *__restrict__ src,
 	const int g = lane >> 4, t = lane & 15;
 	const GroupSmem<CD> sm = group_smem<CD>(smem, g, 16 * pbw);
 	normal_image_tile<CD, RAW>(src, width, height, dst, dxt, nrandom, refine, seed, pbw, block_base, sm,
-	                           (int)blockIdx.x, g, t);
+	                           xcd_local_id((int)blockIdx.x, (int)gridDim.x), g, t);
 }
 
 // More candidates than LDS holds: each workgroup keeps its four blocks'
@@ -243,7 +243,7 @@ __global__ __launch_bounds__(64) void k_normal_wide(const uint32_t *__restrict__
 	const GroupSmem<CD> sm = group_smem<CD>(gmem + (size_t)blockIdx.x * wg_bytes, g, 16 * pbw);
 	const int bw = (width + 3) >> 2, bh = (height + 3) >> 2;
 	const int ntiles = (bw * bh + 3) >> 2;
-	for (int tile = (int)blockIdx.x; tile < ntiles; tile += (int)gridDim.x) {
+	for (int tile = xcd_local_id((int)blockIdx.x, (int)gridDim.x); tile < ntiles; tile += (int)gridDim.x) {
 		normal_image_tile<CD, RAW>(src, width, height, dst, dxt, nrandom, refine, seed, pbw, block_base, sm, tile,
 		                           g, t);
 		__syncthreads();
@@ -385,7 +385,7 @@ __global__ __launch_bounds__(64) void k_normal_batch(const JobDesc *__restrict__
 	const int lane = (int)threadIdx.x;
 	const int g = lane >> 4, t = lane & 15;
 	const GroupSmem<CD> sm = group_smem<CD>(smem, g, 16 * pbw_max);
-	for (int tile = (int)blockIdx.x; tile < ntiles; tile += (int)gridDim.x) {
+	for (int tile = xcd_local_id((int)blockIdx.x, (int)gridDim.x); tile < ntiles; tile += (int)gridDim.x) {
 		normal_batch_tile<CD>(jobs, tile_jobs, nblocks, sm, tile, g, t);
 		__syncthreads();
 	}
@@ -402,7 +402,7 @@ __global__ __launch_bounds__(64) void k_normal_batch_wide(const JobDesc *__restr
 	const int g = lane >> 4, t = lane & 15;
 	const size_t wg_bytes = 4 * group_smem_bytes<CD>(16 * pbw_max);
 	const GroupSmem<CD> sm = group_smem<CD>(gmem + (size_t)blockIdx.x * wg_bytes, g, 16 * pbw_max);
-	for (int tile = (int)blockIdx.x; tile < ntiles; tile += (int)gridDim.x) {
+	for (int tile = xcd_local_id((int)blockIdx.x, (int)gridDim.x); tile < ntiles; tile += (int)gridDim.x) {
 		normal_batch_tile<CD>(jobs, tile_jobs, nblocks, sm, tile, g, t);
 		__syncthreads();
 	}

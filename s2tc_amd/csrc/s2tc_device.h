@@ -38,6 +38,17 @@ S2TC_D uint32_t texel565(uint32_t t)
 }
 S2TC_D int texel_a(uint32_t t) { return (int)(t >> 24); }
 
+// Workgroup i of a launch runs on XCD i % 8, and each of the 8 XCDs has its
+// own L2.  This renumbers the G workgroups so that each XCD holds one
+// contiguous run of ids (a bijection of 0..G-1): consecutive tiles, which
+// share 128-byte lines of the source, then meet in one L2 instead of being
+// fetched from HBM once per XCD.
+S2TC_D int xcd_local_id(int bid, int G)
+{
+	const int q = G >> 3, r = G & 7, x = bid & 7, s = bid >> 3;
+	return x * q + (x < r ? x : r) + s;
+}
+
 // SHRR(a,n) = ((a) + (1 << ((n)-1))) >> (n) with wrap-around (s2tc_algorithm.cpp:215)
 S2TC_D int shrr(int a, int n) { return (int)((uint32_t)a + (1u << (n - 1))) >> n; }
 S2TC_D int wmul(int a, int b) { return (int)((uint32_t)a * (uint32_t)b); }
