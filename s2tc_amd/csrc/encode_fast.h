@@ -56,7 +56,7 @@ struct ColorRefineLane {
 	static S2TC_D uint32_t try_pass(const uint32_t (&tx)[16], const TexFeat<CD> &f, const Masks &mk, uint32_t c0,
 	                                uint32_t c1, uint32_t &score, int &n1, uint32_t &s1)
 	{
-		const F e0 = D::feat(c0), e1 = D::feat(c1);
+		const DistTo<CD> e0(D::feat(c0)), e1(D::feat(c1));
 		uint32_t bits = 0;
 		score = 0;
 		n1 = 0;
@@ -68,7 +68,7 @@ struct ColorRefineLane {
 			const bool transp = TRANS && ((mk.trans >> k) & 1u);
 			const bool use = valid && !transp;
 			const F fk = f.get(tx, k);
-			const int d0 = D::d(fk, e0), d1 = D::d(fk, e1);
+			const int d0 = e0.d(fk), d1 = e1.d(fk);
 			const bool b = d1 < d0;
 			score += use ? (uint32_t)(b ? d1 : d0) : 0u;
 			const bool b1 = use && b;

@@ -130,6 +130,30 @@ template <> struct LinDist<CD_SRGB_MIXED> {
 	}
 };
 
+// The extra and the endpoint features with SRGB_MIXED's luminance either from
+// the table (default) or computed in registers (AR): srgb_get_y is the device
+// code that fills the table, so both give the same values.  The DXT1 quick
+// kernels hold 138-145 VGPRs (three waves per SIMD), too few waves to hide
+// the table's dependent gathers (one per texel and endpoint): they compute
+// the luminance (16384^2 quick mode 0.71 -> 0.50 ms); the DXT3 and DXT5
+// forms are VALU-bound on blocks with varying alpha and keep the table.
+template <int CD, bool AR>
+S2TC_D typename LinDist<CD>::Extra lin_extra(uint32_t q)
+{
+	if constexpr (AR) return -srgb_get_y((int)(q & 0x1fu), (int)((q >> 8) & 0x3fu), (int)((q >> 16) & 0x1fu));
+	else return LinDist<CD>::extra(q);
+}
+template <int CD, bool AR>
+S2TC_D LinF lin_feat565(uint32_t c)
+{
+	if constexpr (AR) {
+		const int y = srgb_get_y(c_r(c), c_g(c), c_b(c));
+		return LinF{-y, c_r(c) * 191 - y, c_b(c) * 191 - y};
+	} else {
+		return LinDist<CD>::feat565(c);
+	}
+}
+
 // Q(F) = 32 ny^2 + 2 u^2 + v^2 of an endpoint (below 2^31 for every distance here)
 S2TC_D int lin_q(const LinF &f) { return 32 * f.ny * f.ny + 2 * f.u * f.u + f.v * f.v; }
 
@@ -177,11 +201,11 @@ struct LinPass {
 	uint32_t tu, tvlo, tvhi;  // YUV
 };
 
-template <int CD>
+template <int CD, bool AR>
 S2TC_D LinPass lin_pass_setup(uint32_t c0, uint32_t c1)
 {
 	typedef LinDist<CD> L;
-	const LinF e0 = L::feat565(c0), e1 = L::feat565(c1);
+	const LinF e0 = lin_feat565<CD, AR>(c0), e1 = lin_feat565<CD, AR>(c1);
 	LinPass p;
 	p.K = lin_q(e1) - lin_q(e0);
 	const int dny = 64 * (e0.ny - e1.ny), du = 4 * (e0.u - e1.u), dv = 2 * (e0.v - e1.v);
@@ -242,11 +266,11 @@ S2TC_D int lin_test(const LinPass &p, uint32_t q, typename LinDist<CD>::Extra ex
 // One s2tc_try_encode_block pass (no score) for endpoints c0, c1: the
 // slot-ordered index bits of the used texels (1 = endpoint 1), n1 and the
 // byte-plane sums of the texels that chose endpoint 1.
-template <int CD>
+template <int CD, bool AR>
 S2TC_D uint32_t lin_pass(const uint32_t (&U)[16], const typename LinDist<CD>::Extra (&ex)[16],
                          const QuadPlanes &P, uint32_t c0, uint32_t c1, int &n1, QSums &s1)
 {
-	const LinPass p = lin_pass_setup<CD>(c0, c1);
+	const LinPass p = lin_pass_setup<CD, AR>(c0, c1);
 	uint32_t bits = 0, m[4];
 #pragma unroll
 	for (int y = 0; y < 4; ++y) {
@@ -264,7 +288,7 @@ S2TC_D uint32_t lin_pass(const uint32_t (&U)[16], const typename LinDist<CD>::Ex
 
 // colour refinement NEVER / ALWAYS (:767-860) with the linear pass; trans3
 // holds index 3 in the slots of DXT1 transparent texels.
-template <int CD, bool TRANS>
+template <int CD, bool TRANS, bool AR>
 S2TC_D uint32_t lin_color_refine(const uint32_t (&U)[16], const typename LinDist<CD>::Extra (&ex)[16],
                                  const QuadPlanes &P, int ntot, uint32_t trans3, int refine, uint32_t &c0,
                                  uint32_t &c1)
@@ -277,10 +301,10 @@ S2TC_D uint32_t lin_color_refine(const uint32_t (&U)[16], const typename LinDist
 			c0 = c1;
 			c1 = t;
 		}
-		return lin_pass<CD>(U, ex, P, c0, c1, n1, s1) | trans3;
+		return lin_pass<CD, AR>(U, ex, P, c0, c1, n1, s1) | trans3;
 	}
 	const QSums tot = plane_sums(P, P.use);
-	uint32_t bits = lin_pass<CD>(U, ex, P, c0, c1, n1, s1);
+	uint32_t bits = lin_pass<CD, AR>(U, ex, P, c0, c1, n1, s1);
 	const QSums s0{tot.r - s1.r, tot.g - s1.g, tot.b - s1.b};
 	quad_eval_ag<1>(ntot - n1, s0, n1, s1, c0, c1);
 	bits |= trans3;
@@ -299,6 +323,7 @@ S2TC_D BlockOut encode_block_lin(const uint32_t (&px)[16], const uint32_t *__res
 	typedef LinDist<CD> L;
 	typedef SrcWord<RAW> S;
 	constexpr bool ALL = FULL && (DXT != DXT1 || OPAQUE);  // every texel counts
+	constexpr bool AR = CD == CD_SRGB_MIXED && DXT == DXT1;
 	uint32_t U[16];
 	typename L::Extra ex[16];
 	QuadPlanes P;
@@ -316,7 +341,7 @@ S2TC_D BlockOut encode_block_lin(const uint32_t (&px)[16], const uint32_t *__res
 		if (ALL) um = ~0u;
 		usem[k] = um;
 		U[k] = q;
-		ex[k] = L::extra(q);
+		ex[k] = lin_extra<CD, AR>(q);
 		// darkest / brightest by distance to black, first occurrence (:900-922)
 		const uint32_t d = (uint32_t)L::dblack(L::feat(q, ex[k]));
 		const uint32_t dx = ALL ? d : d & um;
@@ -365,12 +390,12 @@ S2TC_D BlockOut encode_block_lin(const uint32_t (&px)[16], const uint32_t *__res
 	BlockOut o;
 	if (DXT == DXT1) {
 		const uint32_t trans3 = (valid2 & ~use2) * 3u;
-		const uint32_t bits = lin_color_refine<CD, true>(U, ex, P, ntot, trans3, refine, c0, c1);
+		const uint32_t bits = lin_color_refine<CD, true, AR>(U, ex, P, ntot, trans3, refine, c0, c1);
 		o.w[0] = c0 | (c1 << 16);
 		o.w[1] = bits;
 		o.w[2] = o.w[3] = 0;
 	} else if (DXT == DXT3) {
-		const uint32_t bits = lin_color_refine<CD, false>(U, ex, P, ntot, 0u, refine, c0, c1);
+		const uint32_t bits = lin_color_refine<CD, false, AR>(U, ex, P, ntot, 0u, refine, c0, c1);
 		uint64_t ab = 0;
 #pragma unroll
 		for (int k = 0; k < 16; ++k) {
@@ -383,7 +408,7 @@ S2TC_D BlockOut encode_block_lin(const uint32_t (&px)[16], const uint32_t *__res
 		o.w[2] = c0 | (c1 << 16);
 		o.w[3] = bits;
 	} else {
-		const uint32_t bits = lin_color_refine<CD, false>(U, ex, P, ntot, 0u, refine, c0, c1);
+		const uint32_t bits = lin_color_refine<CD, false, AR>(U, ex, P, ntot, 0u, refine, c0, c1);
 		uint64_t lo;
 		if (FULL && alpha_uniform(px)) {
 			lo = alpha_uniform_half(texel_a(px[0]));

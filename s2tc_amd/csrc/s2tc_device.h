@@ -53,6 +53,7 @@ S2TC_D int xcd_local_id(int bid, int G)
 S2TC_D int shrr(int a, int n) { return (int)((uint32_t)a + (1u << (n - 1))) >> n; }
 S2TC_D int wmul(int a, int b) { return (int)((uint32_t)a * (uint32_t)b); }
 S2TC_D int wadd(int a, int b) { return (int)((uint32_t)a + (uint32_t)b); }
+S2TC_D int wsub(int a, int b) { return (int)((uint32_t)a - (uint32_t)b); }
 // Full-rate 24-bit multiplies for factors known to lie in (-2^23, 2^23):
 // the low 32 bits of a * b (+ c), i.e. the int32 wrap of the reference's
 // expressions.  Written out because the compiler otherwise picks the
@@ -169,6 +170,32 @@ template <> struct Dist<CD_SRGB> {
 		return wadd(wadd(sy, su), sv);
 	}
 };
+// An endpoint e prepared for many distances d(x, e) to it (the texel
+// first, as s2tc_try_encode_block calls ColorDist).  SRGB keeps e's features
+// less the SHRR rounding offsets, so that SHRR(x - e, n) is one subtraction
+// and one shift: (x - (e - 2^(n-1))) >> n, the subtraction wrapping like the
+// reference's int arithmetic.
+template <int CD> struct DistTo {
+	typedef typename Dist<CD>::F F;
+	F e;
+	S2TC_D explicit DistTo(const F &e_) : e(e_) {}
+	S2TC_D int d(const F &x) const { return Dist<CD>::d(x, e); }
+};
+template <> struct DistTo<CD_SRGB> {
+	int y4, y8, u4, u8, v4, v8;
+	S2TC_D explicit DistTo(const FeatI &e)
+	    : y4(wsub(e.x, 4)), y8(wsub(e.x, 8)), u4(wsub(e.y, 4)), u8(wsub(e.y, 8)), v4(wsub(e.z, 4)), v8(wsub(e.z, 8))
+	{
+	}
+	S2TC_D int d(const FeatI &x) const
+	{
+		const int sy = mad_i24(wsub(x.x, y4) >> 3, wsub(x.x, y8) >> 4, 8) >> 4;
+		const int su = mad_i24(wsub(x.y, u4) >> 3, wsub(x.y, u8) >> 4, 128) >> 8;
+		const int sv = mad_i24(wsub(x.z, v4) >> 3, wsub(x.z, v8) >> 4, 256) >> 9;
+		return wadd(wadd(sy, su), sv);
+	}
+};
+
 // srgb_get_y (:285-296): (int)(sqrtf(37*(84r^2+72g^2+28b^2)) + 0.5f); the
 // argument is below 2^24 so the int->float conversion is exact.
 S2TC_D int srgb_get_y(int r, int g, int b)
