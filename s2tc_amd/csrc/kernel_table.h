@@ -11,7 +11,16 @@ struct JobDesc {
 	int dxt, nrandom, refine, raw;  // raw: RGBA8 quantized on load with DITHER_NONE
 	unsigned long long seed;
 	long long block0;     // first block of this job in its class's packed block index space
+	uint32_t bw_magic;    // min(2^32 / bw, 2^32 - 1) for bw = block columns: see div_blocks
+	uint32_t pad;
 };
+
+// The multiplier of div_blocks for bw block columns (bw >= 1).
+inline uint32_t bw_magic(int bw)
+{
+	const unsigned long long m = (1ull << 32) / (unsigned long long)bw;
+	return m > 0xffffffffull ? 0xffffffffu : (uint32_t)m;
+}
 
 // First and last job index a batch tile touches.
 struct TileJobs {

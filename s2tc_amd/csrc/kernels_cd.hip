@@ -38,23 +38,29 @@ S2TC_D uint32_t tx_min16(const uint32_t (&tx)[16])
 }
 
 // Load the 4x4 block (bx, by) into tx[x*4+y]; texels outside w x h are 0.
+// Full blocks of 16-byte aligned rows (vec: width % 4 == 0) take four
+// unconditional 16-byte loads.
 template <int DXT, bool RAW>
 S2TC_D void load_block(const uint32_t *__restrict__ src, int width, int bx, int by, int w, int h, bool vec,
                        uint32_t (&tx)[16])
 {
+	const uint32_t *base = src + (size_t)(4 * by) * (size_t)width + 4 * bx;
+	if (vec && w == 4 && h == 4) {
 #pragma unroll
-	for (int k = 0; k < 16; ++k) tx[k] = 0;
+		for (int y = 0; y < 4; ++y) {
+			const uint4 q = *(const uint4 *)(base + y * width);
+			tx[y] = q.x;
+			tx[4 + y] = q.y;
+			tx[8 + y] = q.z;
+			tx[12 + y] = q.w;
+		}
+	} else {
 #pragma unroll
-	for (int y = 0; y < 4; ++y) {
-		if (y < h) {
-			const uint32_t *row = src + (size_t)(4 * by + y) * (size_t)width + 4 * bx;
-			if (vec && w == 4) {
-				const uint4 q = *(const uint4 *)row;
-				tx[y] = q.x;
-				tx[4 + y] = q.y;
-				tx[8 + y] = q.z;
-				tx[12 + y] = q.w;
-			} else {
+		for (int k = 0; k < 16; ++k) tx[k] = 0;
+#pragma unroll
+		for (int y = 0; y < 4; ++y) {
+			if (y < h) {
+				const uint32_t *row = base + y * width;
 #pragma unroll
 				for (int x = 0; x < 4; ++x)
 					if (x < w) tx[x * 4 + y] = row[x];
@@ -260,6 +266,23 @@ __global__ __launch_bounds__(64) void k_normal_wide(const uint32_t *__restrict__
 // gridDim.x), so the grid size is a free parameter and no counters need
 // resetting between launches.
 
+// by = b / bw, bx = b % bw for 0 <= b < 2^31 with bw and b / bw below 2^23
+// (the batch plan rejects larger images), m = bw_magic(bw) = min(2^32 / bw,
+// 2^32 - 1): b m / 2^32 lies in (b / bw - 1, b / bw], so its floor is the
+// quotient or one less, and one correction step fixes it.  A quarter-rate
+// high multiply and a few full-rate instructions instead of an integer
+// division.
+S2TC_D void div_blocks(int b, int bw, uint32_t m, int &by, int &bx)
+{
+	uint32_t q = __umulhi((uint32_t)b, m);
+	uint32_t r = (uint32_t)b - (uint32_t)mul_i24((int)q, bw);
+	const bool up = r >= (uint32_t)bw;
+	q += up ? 1u : 0u;
+	r -= up ? (uint32_t)bw : 0u;
+	by = (int)q;
+	bx = (int)r;
+}
+
 S2TC_D int find_job(const JobDesc *__restrict__ jobs, int lo, int hi, long long gb)
 {
 	while (lo < hi) {
@@ -281,7 +304,8 @@ __global__ __launch_bounds__(256) void k_fast_batch(const JobDesc *__restrict__ 
 		const JobDesc &j = jobs[find_job(jobs, tj.lo, tj.hi, gb)];
 		const int b = (int)(gb - j.block0);
 		const int bw = (j.width + 3) >> 2;
-		const int by = b / bw, bx = b - by * bw;
+		int by, bx;
+		div_blocks(b, bw, j.bw_magic, by, bx);
 		const int w = min(4, j.width - 4 * bx), h = min(4, j.height - 4 * by);
 		const bool vec = (j.width & 3) == 0;
 		uint32_t tx[16];
@@ -316,7 +340,8 @@ S2TC_D void all16_batch_tiles(const JobDesc *__restrict__ jobs, const TileJobs *
 		const JobDesc &j = jobs[find_job(jobs, tj.lo, tj.hi, gb)];
 		const int b = (int)(gb - j.block0);
 		const int bw = (j.width + 3) >> 2;
-		const int by = b / bw, bx = b - by * bw;
+		int by, bx;
+		div_blocks(b, bw, j.bw_magic, by, bx);
 		const int w = min(4, j.width - 4 * bx), h = min(4, j.height - 4 * by);
 		const bool vec = (j.width & 3) == 0;
 		uint32_t tx[16];
@@ -359,7 +384,8 @@ S2TC_D void normal_batch_tile(const JobDesc *__restrict__ jobs, const TileJobs *
 	const JobDesc &j = jobs[find_job(jobs, tj.lo, tj.hi, gb)];
 	const int b = (int)(gb - j.block0);
 	const int bw = (j.width + 3) >> 2;
-	const int by = b / bw, bx = b - by * bw;
+	int by, bx;
+	div_blocks(b, bw, j.bw_magic, by, bx);
 	const int w = min(4, j.width - 4 * bx), h = min(4, j.height - 4 * by);
 	const int x = t >> 2, y = t & 3;
 	uint32_t tex = 0;

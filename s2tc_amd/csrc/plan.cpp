@@ -51,6 +51,10 @@ int plan_classes(const s2tc_amd_job *jobs, int njobs, std::vector<ClassPlan> &cl
 		int dxt = j.dxt, cd = j.cd, refine = j.refine;
 		normalize_modes(dxt, cd, refine);
 		if (j.width <= 0 || j.height <= 0 || !j.src || !j.dst) return set_error(S2TC_E_ARG, "job %d: bad image", i);
+		// block indices are int and the kernels split them with 24-bit multiplies
+		const long long jbw = (j.width + 3LL) / 4, jbh = (j.height + 3LL) / 4;
+		if (jbw >= (1LL << 23) || jbh >= (1LL << 23) || jbw * jbh >= (1LL << 31))
+			return set_error(S2TC_E_RANGE, "job %d: %d x %d texels exceed the batch limits", i, j.width, j.height);
 		if (j.nrandom > S2TC_MAX_NRANDOM)
 			return set_error(S2TC_E_RANGE, "job %d: nrandom %d exceeds the GPU limit %d", i, j.nrandom,
 			                 S2TC_MAX_NRANDOM);
@@ -86,6 +90,8 @@ int plan_classes(const s2tc_amd_job *jobs, int njobs, std::vector<ClassPlan> &cl
 		d.raw = j.src_kind == S2TC_SRC_RGBA8 ? 1 : 0;
 		d.seed = j.seed;
 		d.block0 = c.nblocks;
+		d.bw_magic = bw_magic((j.width + 3) / 4);
+		d.pad = 0;
 		const long long nb = (long long)((j.width + 3) / 4) * ((j.height + 3) / 4);
 		c.nblocks += nb;
 		c.texels += (long long)j.width * j.height;

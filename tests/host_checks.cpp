@@ -1,11 +1,13 @@
 // Host-only checks of library helpers, compiled with g++ against the
 // library's own headers by tests/test_host_checks.py.  Prints "ok" and exits
 // 0 when every check passes; otherwise prints the failing case and exits 1.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
 
+#include "kernel_table.h"
 #include "staging.h"
 
 using s2tc::stage_slice;
@@ -38,6 +40,49 @@ static bool fs_mulhi_division_exact()
 				printf("fs division: d %u v %u\n", d, v);
 				return false;
 			}
+	}
+	return true;
+}
+
+// The batch kernels split a job's block index b into (b / bw, b % bw) with
+// the multiplier bw_magic(bw): q = (b * m) >> 32, then one upward correction
+// when b - q bw >= bw.  Checked against the division for every b below 2^20
+// at bw 1..256, and around multiples of bw (and at the top of the range) for
+// every bw up to 2048 and larger ones up to the batch limit 2^23 - 1.
+static bool block_row_division(unsigned b, unsigned bw)
+{
+	const uint32_t m = bw_magic((int)bw);
+	uint32_t q = (uint32_t)(((unsigned long long)b * m) >> 32);
+	uint32_t r = b - q * bw;
+	if (r >= bw) {
+		++q;
+		r -= bw;
+	}
+	if (q != b / bw || r != b % bw) {
+		printf("block division: b %u bw %u -> %u %u\n", b, bw, q, r);
+		return false;
+	}
+	return true;
+}
+
+static bool block_row_division_exact()
+{
+	for (unsigned bw = 1; bw <= 256; ++bw)
+		for (unsigned b = 0; b < (1u << 20); ++b)
+			if (!block_row_division(b, bw)) return false;
+	const unsigned lim = (1u << 31) - 1;
+	std::vector<unsigned> bws;
+	for (unsigned bw = 257; bw <= 2048; ++bw) bws.push_back(bw);
+	for (unsigned bw : {4096u, 65535u, 1000003u, (1u << 23) - 1u}) bws.push_back(bw);
+	for (unsigned bw : bws) {
+		// rows q with q < 2^23 and q * bw + bw - 1 <= lim
+		const unsigned long long qmax = std::min<unsigned long long>((1ull << 23) - 1, (lim - (bw - 1)) / bw);
+		for (unsigned long long q = 0; q <= qmax; q += (q < 4096 ? 1 : 99991)) {
+			const unsigned long long b0 = q * bw;
+			for (unsigned long long b : {b0, b0 + 1, b0 + bw / 2, b0 + bw - 1})
+				if (!block_row_division((unsigned)b, bw)) return false;
+		}
+		if (!block_row_division((unsigned)std::min<unsigned long long>(lim, qmax * bw + bw - 1), bw)) return false;
 	}
 	return true;
 }
@@ -141,6 +186,7 @@ static bool linear_test_weight_ranges()
 int main()
 {
 	if (!fs_mulhi_division_exact()) return 1;
+	if (!block_row_division_exact()) return 1;
 	if (!linear_test_identities()) return 1;
 	if (!linear_test_weight_ranges()) return 1;
 	std::vector<size_t> lens;
