@@ -227,3 +227,28 @@ def test_autotuned_forms_mixed_batch(gpu, oracle):
             b.set_form(quick[0], gpu.FORM_ROW16)
     b.destroy()
     imgs.free()
+
+
+def test_batch_rejects_oversized_jobs(gpu):
+    """A job with 2^23 or more block rows or columns is refused when the batch
+    is planned (S2TC_E_RANGE), before any memory is read or any kernel runs;
+    one just inside the limit is planned (its buffers are full size, so even
+    a launch would stay in bounds)."""
+    import ctypes
+    L = gpu._bind_batch(gpu.lib())
+    L.s2tc_amd_batch_destroy.argtypes = [ctypes.c_void_p]
+    wmax = 4 * ((1 << 23) - 1)
+    src = gpu.DeviceBuffer(wmax * 4 * 4)
+    dst = gpu.DeviceBuffer(((1 << 23) - 1) * 8)
+    for w, h, ok in ((4 << 23, 4, False), (4, 4 << 23, False), (wmax, 4, True)):
+        print(f"case plan {w}x{h}", flush=True)
+        job = gpu.Job(src.ptr, dst.ptr, w, h, gpu.SRC_QUANTIZED, 0, 5, -1, 1, 0)
+        h_out = ctypes.c_void_p()
+        rc = L.s2tc_amd_batch_create(ctypes.byref(job), 1, ctypes.byref(h_out))
+        if ok:
+            assert rc == 0, gpu.lib().s2tc_amd_last_error()
+            L.s2tc_amd_batch_destroy(h_out.value)
+        else:
+            assert rc == -4 and not h_out.value, (rc, gpu.lib().s2tc_amd_last_error())
+    src.free()
+    dst.free()
