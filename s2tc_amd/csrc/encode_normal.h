@@ -221,11 +221,17 @@ template <int END, bool WIDE> struct KeyStepsDiag<END, END, WIDE> {
 	static S2TC_D void run(const int (&)[16], int, uint32_t, typename ScanKeyT<WIDE>::T &) {}
 };
 
-template <bool WIDE>
-S2TC_D uint32_t reduce_rows_key(const int *rows, int t, int m, int pbw)
+// MAP: the rows are those of a candidate list without repeated colours,
+// entry a of `ul` = {original index | colour << 16, original index of the
+// colour's second occurrence (all-ones if none), row sum}; pair orders are
+// formed from the original indices, and each colour that occurs twice also
+// offers its pair (first, second), whose sum is its row sum (min(x, x) = x).
+template <bool WIDE, bool MAP = false>
+S2TC_D uint32_t reduce_rows_key(const int *rows, int t, int m, int pbw, const uint32_t *ul = nullptr)
 {
 	typedef typename ScanKeyT<WIDE>::T K;
 	ScanKey st{0xffffffffu, 0xffffffffu, -1};
+	const auto orig = [&](uint32_t a) -> uint32_t { return MAP ? (ul[3 * a] & 0xffffu) : a; };
 	for (int p = 0; p < pbw; ++p) {
 		int X[16];
 		load_row(rows, 16 * p + t, X);
@@ -235,15 +241,21 @@ S2TC_D uint32_t reduce_rows_key(const int *rows, int t, int m, int pbw)
 			K kmin = ~(K)0;
 			KeyStepsDiag<1, 9, WIDE>::run(X, t, irow ? (uint32_t)t : 0xffffffffu, kmin);
 			const uint32_t lo = 16u * p + (uint32_t)((kmin >> 4) & 15u), hi = 16u * p + (uint32_t)(kmin & 15u);
-			merge_key((uint32_t)(kmin >> 8), (lo << 16) | hi, irow && kmin != ~(K)0, st);
+			const bool ok = irow && kmin != ~(K)0;
+			merge_key((uint32_t)(kmin >> 8), ok ? (orig(lo) << 16) | orig(hi) : 0u, ok, st);
 		}
 		for (int q = p + 1; q < pbw; ++q) {
 			int Y[16];
 			load_row(rows, 16 * q + t, Y);
 			K kmin = ~(K)0;
 			KeyStepsOff<0, 16, WIDE>::run(X, Y, 16 * q + t < m ? (uint32_t)t : 0xffffffffu, kmin);
-			const bool ok = WIDE ? (kmin >> 32) < 8u : kmin != ~(K)0;
-			merge_key((uint32_t)(kmin >> 4), ((uint32_t)i << 16) | (16u * q + (uint32_t)(kmin & 15u)), irow && ok, st);
+			const bool ok = irow && (WIDE ? (kmin >> 32) < 8u : kmin != ~(K)0);
+			const uint32_t j = 16u * q + (uint32_t)(kmin & 15u);
+			merge_key((uint32_t)(kmin >> 4), ok ? (orig((uint32_t)i) << 16) | orig(j) : 0u, ok, st);
+		}
+		if (MAP && irow) {
+			const uint32_t second = ul[3 * i + 1];
+			merge_key(ul[3 * i + 2], (orig((uint32_t)i) << 16) | second, second != 0xffffffffu, st);
 		}
 	}
 	row_min_key(st.hi, st.lo);
@@ -440,7 +452,22 @@ S2TC_D uint32_t alpha_refine_group(int a, bool valid, int refine, int &a0, int &
 // texel (0 when outside the block), t: lane within the row (texel x-major
 // index), pbw: row blocks = ceil((16 + max(nrandom, 0)) / 16) (kernel-uniform),
 // bkey: PRNG key of the block.  Lane 0 of the row gets the block in o.
-template <int CD>
+// Distances whose pair scan is a plain minimum (all but SRGB, whose wrapped
+// negative sums make the fold depend on the order of every pair).
+template <int CD> struct PlainScan {
+	static const bool value = DistBound<CD>::value != 0 || IsSymmetric<CD>::value;
+};
+
+// DEDUP (group memory in LDS): the colour scan runs over the candidate list
+// without repeated colours.  A pair's sum depends only on its two colours
+// (every row is a function of its candidate's colour: d(c, c) = 0 and the
+// distances are symmetric), so the first pair in the reference's (i, j)
+// order with the least sum is either (first A, first B) for two colours
+// A != B (every other pair of those colours comes later in that order) or
+// (first A, second A), whose sum is A's row sum.  Small colour boxes (smooth
+// blocks: the random candidates fall on few colours) shrink the scan from
+// m (m - 1) / 2 pairs to about u^2 / 2 for u distinct colours.
+template <int CD, bool DEDUP = false>
 S2TC_D void encode_block_normal(uint32_t tex, int t, int w, int h, int dxt, int nrandom, int refine, uint32_t bkey,
                                 int pbw, const GroupSmem<CD> &sm, BlockOut &o)
 {
@@ -477,13 +504,15 @@ S2TC_D void encode_block_normal(uint32_t tex, int t, int w, int h, int dxt, int 
 	}
 	__syncthreads();
 	int m = n;
+	int minr = 0, ming = 0, minb = 0, lr = 1, lg = 1, lb = 1;  // the colour box of the random candidates
 	if (nrandom > 0) {  // (:962-993)
 		int r = c_r(v), g = c_g(v), b = c_b(v);
-		int minr = row_min(use ? r : 255), ming = row_min(use ? g : 255), minb = row_min(use ? b : 255);
+		minr = row_min(use ? r : 255), ming = row_min(use ? g : 255), minb = row_min(use ? b : 255);
 		int maxr = row_max(use ? r : 0), maxg = row_max(use ? g : 0), maxb = row_max(use ? b : 0);
 		int mina = row_min(use ? alpha : 255), maxa = row_max(use ? alpha : 0);
 		if (none) minr = ming = minb = maxr = maxg = maxb = mina = maxa = 0;
-		const int lr = maxr - minr + 1, lg = maxg - ming + 1, lb = maxb - minb + 1, la = maxa - mina + 1;
+		lr = maxr - minr + 1, lg = maxg - ming + 1, lb = maxb - minb + 1;
+		const int la = maxa - mina + 1;
 		const int dpc = dxt == DXT5 ? 4 : 3;
 		for (int c = t; c < nrandom; c += 16) {
 			const uint32_t k = (uint32_t)(c * dpc);
@@ -501,15 +530,81 @@ S2TC_D void encode_block_normal(uint32_t tex, int t, int w, int h, int dxt, int 
 	}
 	__syncthreads();
 
+	// Repeated colours (DEDUP).  In this block's row memory (16 pbw rows of
+	// S2TC_RS ints): T1[V] and T2[V] over the colour box (V <= 256 cells) take
+	// the first and second candidate index of each colour by atomic minimum,
+	// then the list of first occurrences goes to the last 3 m ints, which the
+	// compacted rows (written from the top) must not reach.  Every lane of the
+	// workgroup passes the barriers; blocks that do not qualify skip the work.
+	int mu = m, pbu = pbw;
+	bool dd = false;
+	uint32_t *ul = nullptr;
+	if constexpr (DEDUP && PlainScan<CD>::value) {
+		const int cap = 16 * pbw * S2TC_RS;
+		const int V = lr * lg * lb;
+		dd = !cuni && nrandom > 0 && pbw >= 2 && V <= 256 && 2 * V + 3 * m <= cap;
+		int *T1 = sm.rows, *T2 = sm.rows + V;
+		ul = (uint32_t *)(sm.rows + cap - 3 * m);
+		const auto cell = [&](uint32_t c) {
+			return ((c_r(c) - minr) * lg + (c_g(c) - ming)) * lb + (c_b(c) - minb);
+		};
+		if (dd)
+			for (int z = t; z < V; z += 16) T1[z] = T2[z] = 0x7fffffff;
+		__syncthreads();
+		if (dd)
+			for (int i = t; i < m; i += 16) atomicMin(&T1[cell(sm.cand[i] & 0xffffu)], i);
+		__syncthreads();
+		if (dd)
+			for (int i = t; i < m; i += 16) {
+				const int z = cell(sm.cand[i] & 0xffffu);
+				if (T1[z] != i) atomicMin(&T2[z], i);
+			}
+		__syncthreads();
+		if (dd) {
+			int base = 0;
+			for (int p = 0; p < pbw; ++p) {
+				const int i = 16 * p + t;
+				bool first = false;
+				uint32_t c = 0u;
+				int sec = 0x7fffffff;
+				if (i < m) {
+					c = sm.cand[i] & 0xffffu;
+					const int z = cell(c);
+					first = T1[z] == i;
+					sec = T2[z];
+				}
+				const uint32_t fm = (uint32_t)(__ballot(first) >> gshift) & 0xffffu;
+				if (first) {
+					const int a = base + __builtin_popcount(fm & ((1u << t) - 1u));
+					ul[3 * a] = (uint32_t)i | (c << 16);
+					ul[3 * a + 1] = sec == 0x7fffffff ? 0xffffffffu : (uint32_t)sec;
+				}
+				base += __builtin_popcount(fm);
+			}
+			mu = base;
+			pbu = (mu + 15) >> 4;
+			// the compacted rows (16 pbu of them are written) must stay clear of
+			// the list, and must save work
+			if (pbu >= pbw || 16 * pbu * S2TC_RS + 3 * m > cap) {
+				dd = false;
+				mu = m;
+				pbu = pbw;
+			}
+		}
+		__syncthreads();
+	}
+
 	// features of the first 16 candidates (the columns)
 	if (t < m) sm.feat[t] = D::feat(sm.cand[t] & 0xffffu);
 	__syncthreads();
 
-	// dists[i][k] rows (:373-392): square part dist(c[min], c[max]), box part dist(c[i], c[k])
-	for (int p = 0; p < (cuni ? 0 : pbw); ++p) {
+	// dists[i][k] rows (:373-392): square part dist(c[min], c[max]), box part
+	// dist(c[i], c[k]); with DEDUP the rows of the list of first occurrences,
+	// each with its row sum
+	for (int p = 0; p < (cuni ? 0 : pbu); ++p) {
 		const int i = 16 * p + t;
-		const bool irow = i < m;
-		const F fi = D::feat(sm.cand[irow ? i : 0] & 0xffffu);
+		const bool irow = i < mu;
+		const F fi = D::feat(dd ? (irow ? ul[3 * i] >> 16 : 0u) : sm.cand[irow ? i : 0] & 0xffffu);
 		int dv[16];
 #pragma unroll
 		for (int k = 0; k < 16; ++k) {
@@ -522,6 +617,12 @@ S2TC_D void encode_block_normal(uint32_t tex, int t, int w, int h, int dxt, int 
 		int4 *rp = (int4 *)(sm.rows + i * S2TC_RS);
 #pragma unroll
 		for (int k = 0; k < 4; ++k) rp[k] = make_int4(dv[4 * k], dv[4 * k + 1], dv[4 * k + 2], dv[4 * k + 3]);
+		if (dd && irow) {
+			uint32_t rs = 0;
+#pragma unroll
+			for (int k = 0; k < 16; ++k) rs += (uint32_t)dv[k];
+			ul[3 * i + 2] = rs;
+		}
 	}
 	__syncthreads();
 	uint32_t c0, c1;
@@ -529,9 +630,15 @@ S2TC_D void encode_block_normal(uint32_t tex, int t, int w, int h, int dxt, int 
 		c0 = c1 = none ? 0u : (uint32_t)vmin;
 	} else {
 		uint32_t cbest;
-		if constexpr (DistBound<CD>::value != 0) cbest = reduce_rows_key<false>(sm.rows, t, m, pbw);
-		else if constexpr (IsSymmetric<CD>::value) cbest = reduce_rows_key<true>(sm.rows, t, m, pbw);
-		else cbest = reduce_rows<true>(sm.rows, t, m, pbw);
+		if constexpr (DistBound<CD>::value != 0) {
+			if (dd) cbest = reduce_rows_key<false, true>(sm.rows, t, mu, pbu, ul);
+			else cbest = reduce_rows_key<false>(sm.rows, t, m, pbw);
+		} else if constexpr (IsSymmetric<CD>::value) {
+			if (dd) cbest = reduce_rows_key<true, true>(sm.rows, t, mu, pbu, ul);
+			else cbest = reduce_rows_key<true>(sm.rows, t, m, pbw);
+		} else {
+			cbest = reduce_rows<true>(sm.rows, t, m, pbw);
+		}
 		const int ci = min((int)(cbest >> 16), m - 1), cj = min((int)(cbest & 0xffffu), m - 1);
 		c0 = sm.cand[ci] & 0xffffu;
 		c1 = sm.cand[cj] & 0xffffu;

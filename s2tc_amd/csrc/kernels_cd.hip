@@ -191,7 +191,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(All16Waves<
 
 // MODE_NORMAL over one image: 16 lanes per block, 4 blocks per 64-lane
 // workgroup; tile `tile` covers raster blocks 4*tile .. 4*tile+3.
-template <int CD, bool RAW>
+template <int CD, bool RAW, bool DEDUP>
 S2TC_D void normal_image_tile(const uint32_t *__restrict__ src, int width, int height, uint8_t *__restrict__ dst,
                               int dxt, int nrandom, int refine, unsigned long long seed, int pbw,
                               unsigned block_base, const GroupSmem<CD> &sm, int tile, int g, int t)
@@ -210,7 +210,7 @@ S2TC_D void normal_image_tile(const uint32_t *__restrict__ src, int width, int h
 	}
 	const uint32_t bkey = s2tc_prng_block_key((uint64_t)seed, block_base + (uint32_t)b);
 	BlockOut o;
-	encode_block_normal<CD>(tex, t, w, h, dxt, nrandom, refine, bkey, pbw, sm, o);
+	encode_block_normal<CD, DEDUP>(tex, t, w, h, dxt, nrandom, refine, bkey, pbw, sm, o);
 	if (have && t == 0) {
 		if (dxt == DXT1) store_block<DXT1>(dst, (size_t)b, o);
 		else store_block<DXT5>(dst, (size_t)b, o);
@@ -228,8 +228,8 @@ __global__ __launch_bounds__(64) void k_normal(const uint32_t *__restrict__ src,
 	const int lane = (int)threadIdx.x;
 	const int g = lane >> 4, t = lane & 15;
 	const GroupSmem<CD> sm = group_smem<CD>(smem, g, 16 * pbw);
-	normal_image_tile<CD, RAW>(src, width, height, dst, dxt, nrandom, refine, seed, pbw, block_base, sm,
-	                           xcd_local_id((int)blockIdx.x, (int)gridDim.x), g, t);
+	normal_image_tile<CD, RAW, true>(src, width, height, dst, dxt, nrandom, refine, seed, pbw, block_base, sm,
+	                                 xcd_local_id((int)blockIdx.x, (int)gridDim.x), g, t);
 }
 
 // More candidates than LDS holds: each workgroup keeps its four blocks'
@@ -250,8 +250,8 @@ __global__ __launch_bounds__(64) void k_normal_wide(const uint32_t *__restrict__
 	const int bw = (width + 3) >> 2, bh = (height + 3) >> 2;
 	const int ntiles = (bw * bh + 3) >> 2;
 	for (int tile = xcd_local_id((int)blockIdx.x, (int)gridDim.x); tile < ntiles; tile += (int)gridDim.x) {
-		normal_image_tile<CD, RAW>(src, width, height, dst, dxt, nrandom, refine, seed, pbw, block_base, sm, tile,
-		                           g, t);
+		normal_image_tile<CD, RAW, false>(src, width, height, dst, dxt, nrandom, refine, seed, pbw, block_base, sm,
+		                                  tile, g, t);
 		__syncthreads();
 	}
 }
@@ -373,7 +373,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 	all16_batch_tiles<CD, DXT, REFINE>(jobs, tile_jobs, nblocks, ntiles);
 }
 
-template <int CD>
+template <int CD, bool DEDUP>
 S2TC_D void normal_batch_tile(const JobDesc *__restrict__ jobs, const TileJobs *__restrict__ tile_jobs,
                               long long nblocks, const GroupSmem<CD> &sm, int tile, int g, int t)
 {
@@ -396,7 +396,7 @@ S2TC_D void normal_batch_tile(const JobDesc *__restrict__ jobs, const TileJobs *
 	const uint32_t bkey = s2tc_prng_block_key((uint64_t)j.seed, (uint32_t)b);
 	const int pbw = (16 + (j.nrandom > 0 ? j.nrandom : 0) + 15) / 16;
 	BlockOut o;
-	encode_block_normal<CD>(tex, t, w, h, j.dxt, j.nrandom, j.refine, bkey, pbw, sm, o);
+	encode_block_normal<CD, DEDUP>(tex, t, w, h, j.dxt, j.nrandom, j.refine, bkey, pbw, sm, o);
 	if (have && t == 0) {
 		if (j.dxt == DXT1) store_block<DXT1>(j.dst, (size_t)b, o);
 		else store_block<DXT5>(j.dst, (size_t)b, o);
@@ -412,7 +412,7 @@ __global__ __launch_bounds__(64) void k_normal_batch(const JobDesc *__restrict__
 	const int g = lane >> 4, t = lane & 15;
 	const GroupSmem<CD> sm = group_smem<CD>(smem, g, 16 * pbw_max);
 	for (int tile = xcd_local_id((int)blockIdx.x, (int)gridDim.x); tile < ntiles; tile += (int)gridDim.x) {
-		normal_batch_tile<CD>(jobs, tile_jobs, nblocks, sm, tile, g, t);
+		normal_batch_tile<CD, true>(jobs, tile_jobs, nblocks, sm, tile, g, t);
 		__syncthreads();
 	}
 }
@@ -429,7 +429,7 @@ __global__ __launch_bounds__(64) void k_normal_batch_wide(const JobDesc *__restr
 	const size_t wg_bytes = 4 * group_smem_bytes<CD>(16 * pbw_max);
 	const GroupSmem<CD> sm = group_smem<CD>(gmem + (size_t)blockIdx.x * wg_bytes, g, 16 * pbw_max);
 	for (int tile = xcd_local_id((int)blockIdx.x, (int)gridDim.x); tile < ntiles; tile += (int)gridDim.x) {
-		normal_batch_tile<CD>(jobs, tile_jobs, nblocks, sm, tile, g, t);
+		normal_batch_tile<CD, false>(jobs, tile_jobs, nblocks, sm, tile, g, t);
 		__syncthreads();
 	}
 }

@@ -73,6 +73,28 @@ def flat_blocks(rng, w, h, dxt):
     return np.ascontiguousarray(q[:h, :w]).reshape(-1)
 
 
+def small_box_blocks(rng, w, h, dxt):
+    """Quantized texels whose blocks span small colour boxes (each channel
+    within 1, 2, 4 or 8 steps of a base), so that random candidates repeat
+    colours; alpha varies within 4 steps (DXT5), and 1/8 of the blocks carry
+    transparent texels (DXT1)."""
+    ab = (1 << ALPHABITS[dxt]) - 1
+    bw, bh = (w + 3) // 4, (h + 3) // 4
+    q = np.zeros((bh * 4, bw * 4, 4), np.uint8)
+    for by in range(bh):
+        for bx in range(bw):
+            b = q[4 * by:4 * by + 4, 4 * bx:4 * bx + 4]
+            span = int(rng.choice([1, 2, 4, 8]))
+            for c, top in enumerate((32, 64, 32)):
+                base = int(rng.integers(0, top - span + 1))
+                b[..., c] = base + rng.integers(0, span, (4, 4))
+            base_a = int(rng.integers(0, max(1, ab - 3) + 1))
+            b[..., 3] = np.minimum(ab, base_a + rng.integers(0, 4, (4, 4))) if dxt else ab
+            if dxt == 0 and rng.random() < 0.125:
+                b[..., 3] = np.where(rng.random((4, 4)) < 0.3, 0, ab)
+    return np.ascontiguousarray(q[:h, :w]).reshape(-1)
+
+
 def special_blocks(dxt):
     """A 4-block-row image of hand-picked 4x4 blocks (returns q, w, h)."""
     ab = (1 << ALPHABITS[dxt]) - 1

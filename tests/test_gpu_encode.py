@@ -75,6 +75,19 @@ def test_one_colour_blocks(gpu, oracle, cd):
             _compare(gpu, oracle, q, w, h, dxt, cd, nr, refine, 5 + cd, "flat")
 
 
+@pytest.mark.parametrize("cd", range(8))
+def test_random_candidates_small_boxes(gpu, oracle, cd):
+    """Random candidates drawn from small colour boxes repeat colours: the
+    scan over the list without repeats (first occurrences and each repeated
+    colour's first pair) must pick the reference's pair, at candidate counts
+    where the compacted list is short (m = 16 + 5 .. 16 + 200), with ties."""
+    rng = np.random.default_rng(400 + cd)
+    for (w, h), dxt in itertools.product([(64, 32), (21, 10)], range(3)):
+        q = imgutil.small_box_blocks(rng, w, h, dxt)
+        for nr, refine in itertools.product([5, 64, 200], [0, 2]):
+            _compare(gpu, oracle, q, w, h, dxt, cd, nr, refine, 9 + cd, "smallbox")
+
+
 def test_raw_rgba8_source(gpu, oracle):
     """Fused DITHER_NONE quantization on load equals rgb565_image + encode."""
     rng = np.random.default_rng(5)

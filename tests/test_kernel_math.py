@@ -174,3 +174,49 @@ def test_pair_sums_of_unbounded_distances_never_wrap(oracle):
                 best = max(best, oracle.color_dist(cd, (int(r[i]), int(g[i]), int(b[i])),
                                                    (int(r[j]), int(g[j]), int(b[j]))))
         assert 16 * (best + 2) < 2 ** 31, (cd, best)
+
+
+def test_candidate_dedup_equals_full_scan():
+    """The random-candidate kernel's colour scan over the list without
+    repeated colours (first occurrences, plus each repeated colour's pair
+    (first, second) scored by its row sum) picks the same pair as
+    reduce_colors_inplace's full scan (:393-410: first pair in (i, j) order
+    with the least sum), for symmetric distances with d(c, c) = 0.  Random
+    lists with many repeats and small distance ranges, so that ties are
+    common."""
+    rng = np.random.default_rng(11)
+    for trial in range(2000):
+        n = int(rng.integers(2, 17))
+        m = n + int(rng.integers(1, 65))
+        ncol = int(rng.integers(1, 9))
+        cols = rng.integers(0, ncol, m)
+        # a symmetric distance between colours with a zero diagonal, small values for ties
+        dc = rng.integers(0, 4, (ncol, ncol))
+        dc = np.triu(dc, 1)
+        dc = dc + dc.T
+        rows = dc[cols][:, cols[:n]]          # dists[i][k] = d(c_i, c_k), k < n
+        # the reference's fold
+        best, bi, bj = -1, 0, 1
+        for i in range(m):
+            for j in range(i + 1, m):
+                s = int(np.minimum(rows[i], rows[j]).sum())
+                if best < 0 or s < best:
+                    best, bi, bj = s, i, j
+        # the kernel's: keys (sum, i << 16 | j) over first-occurrence pairs and (first, second) pairs
+        first, second = {}, {}
+        for i, c in enumerate(cols):
+            if c not in first:
+                first[c] = i
+            elif c not in second:
+                second[c] = i
+        ul = sorted(first.values())
+        keys = []
+        for a in range(len(ul)):
+            for b in range(a + 1, len(ul)):
+                s = int(np.minimum(rows[ul[a]], rows[ul[b]]).sum())
+                keys.append((s, ul[a] << 16 | ul[b]))
+            c = cols[ul[a]]
+            if c in second:
+                keys.append((int(rows[ul[a]].sum()), ul[a] << 16 | second[c]))
+        s, order = min(keys)
+        assert (order >> 16, order & 0xffff) == (bi, bj) and s == best, (trial, n, m, ncol)
