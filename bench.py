@@ -103,8 +103,8 @@ def cpu_baseline(items, imgs=None, stride=5):
     """The compiled reference (oracle/_ref: the unmodified encoder sources),
     threads over all host cores, on every stride-th job of the workload (a
     fixed 1-in-`stride` sample: one pass over the full 16467-job list takes
-    about 11 s on 16 cores, and 1 warm-up + 5 timed passes of it would hold
-    the GPU box for over a minute).  Quantization (DITHER_NONE, as the GPU
+    about 11 s on 16 cores, and 1 warm-up + 5 timed passes of it would keep
+    the benchmark running for over a minute).  Quantization (DITHER_NONE, as the GPU
     path) is done before timing; the timed region is the reference's block
     loop only.  1 warm-up pass, median of 5."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
