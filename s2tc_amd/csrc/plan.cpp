@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 
 namespace s2tc {
 
@@ -188,6 +189,38 @@ void wide_launch_shape(const KernelTable &kt, int pbw, long long ntiles, int ncu
 	bytes = wg * (size_t)grid;
 }
 
+// Workgroups of `kernel` that one CU holds at once with `smem` bytes of
+// dynamic LDS each (registers, LDS and wave slots), cached per shape.
+static int resident_per_cu(const void *kernel, int block, size_t smem)
+{
+	struct Entry {
+		const void *k;
+		int block;
+		size_t smem;
+		int n;
+	};
+	static std::mutex mu;
+	static std::vector<Entry> cache;
+	std::lock_guard<std::mutex> lock(mu);
+	for (const Entry &e : cache)
+		if (e.k == kernel && e.block == block && e.smem == smem) return e.n;
+	int n = 0;
+	if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, block, smem) != hipSuccess || n < 1) n = 0;
+	cache.push_back(Entry{kernel, block, smem, n});
+	return n;
+}
+
+// Grid of a persistent launch: `want` workgroups, at most as many as the
+// device holds at once.  The kernels stride over their tiles by the grid
+// size, so a workgroup that has to wait for a free slot would run its whole
+// share after the others, stretching the launch.
+static int persistent_grid(const void *kernel, int block, size_t smem, long long want, int ncu)
+{
+	const int r = resident_per_cu(kernel, block, smem);
+	if (r > 0) want = std::min(want, (long long)r * ncu);
+	return (int)std::max(1LL, want);
+}
+
 int plan_launch_class(const ClassPlan &c, int wg_per_cu, int ncu, hipStream_t st)
 {
 	long long nb = c.nblocks;
@@ -208,18 +241,22 @@ int plan_launch_class(const ClassPlan &c, int wg_per_cu, int ncu, hipStream_t st
 		int pbw = 1, nt4 = c.ntiles4;
 		const TileJobs *t4 = c.d_tiles4;
 		if (!t4 || !c.forms[FORM_ROW16]) return set_error(S2TC_E_ARG, "class has no 16-lane form");
-		const int grid4 = wg_per_cu <= 0 ? nt4 : std::min(nt4, 4 * wg_per_cu * ncu);
+		const int grid4 = wg_per_cu <= 0 ? nt4
+		                                 : persistent_grid(c.forms[FORM_ROW16], 64, c.smem4,
+		                                                   std::min(nt4, 4 * wg_per_cu * ncu), ncu);
 		void *args[] = {(void *)&jobs, (void *)&t4, (void *)&nb, (void *)&nt4, (void *)&pbw};
 		return hip_check(hipLaunchKernel(c.forms[FORM_ROW16], dim3(grid4), dim3(64), args, c.smem4, st),
 		                 "launch k_normal_batch (all-16 jobs)");
 	}
 	if (c.normal) {
 		int pbw = c.pbw_max;
+		if (wg_per_cu > 0) grid = persistent_grid(c.kernel, 64, c.smem, grid, ncu);
 		void *args[] = {(void *)&jobs, (void *)&tiles, (void *)&nb, (void *)&ntiles, (void *)&pbw};
 		return hip_check(hipLaunchKernel(c.kernel, dim3(grid), dim3(64), args, c.smem, st), "launch k_normal_batch");
 	}
 	void *args[] = {(void *)&jobs, (void *)&tiles, (void *)&nb, (void *)&ntiles};
 	const void *k = c.all16 && c.forms[c.form] ? c.forms[c.form] : c.kernel;
+	if (wg_per_cu > 0) grid = persistent_grid(k, 256, 0, grid, ncu);
 	return hip_check(hipLaunchKernel(k, dim3(grid), dim3(256), args, 0, st),
 	                 c.all16 ? "launch k_all16_batch" : "launch k_fast_batch");
 }
