@@ -248,6 +248,17 @@ typedef struct {
 S2TC_API int s2tc_amd_serve(const s2tc_amd_job *jobs, int njobs, const s2tc_amd_request *reqs, int nreq,
                             const s2tc_amd_serve_opts *opts, double *latency_ms, s2tc_amd_serve_stats *stats);
 
+/* ---- test hook ---- */
+
+/* No counterpart in the reference; for tests only.  The pair scan of the
+ * random-candidate encoder under RGB, YUV and SRGB_MIXED keys its pair sums
+ * saturated at a cap (2^28 - 2), and a block whose least sum reaches the cap
+ * scans again with exact 64-bit keys, so the output never depends on the
+ * cap.  This sets the cap on the current device (clamped to 2^28 - 2, which
+ * 0xffffffff restores); a low cap sends most blocks through the exact
+ * rescan.  Call with no encode in flight. */
+S2TC_API int s2tc_amd_debug_set_scan_cap(unsigned cap);
+
 #ifdef __cplusplus
 }
 #endif

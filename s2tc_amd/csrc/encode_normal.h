@@ -162,21 +162,50 @@ S2TC_D void merge_key(uint32_t s, uint32_t order, bool ok, ScanKey &st)
 
 // The pair scan for rows of non-negative values whose 16-term sums do not
 // wrap (every distance but SRGB, and DXT5 alpha).  Within one row-block pair
-// a lane folds its pairs into one key per step: (sum << 4 | b) for the
-// partner b of block q > p (the pair order is i << 16 | 16q + b, so the
-// smallest key is the smallest (sum, order)), (sum << 8 | lo << 4 | hi) for
-// pairs inside block p.  The partner's index travels through the DPP row
-// with its row values, all-ones when its row is outside the candidates, which
-// makes the key of an invalid pair larger than every valid one.  Keys are 32
-// bits when the sums stay below 2^23 (WIDE false: bounded distances, alpha),
-// 64 bits otherwise (sums below 2^31).  The running (sum, order) minimum is
-// merged once per row-block pair.
-template <bool WIDE> struct ScanKeyT { typedef uint32_t T; };
-template <> struct ScanKeyT<true> { typedef uint64_t T; };
+// a lane folds its pairs into one key per step, (sum << 4 | b) for the
+// partner b (lane within the partner's row block).  Off the diagonal the
+// pair order is i << 16 | 16q + b; on it, the pairs (t, b) of lane t come in
+// the order (min, max), which grows with b (partners below t first, each
+// meeting t as the first of the pair).  So the smallest key is the smallest
+// (sum, order).  The partner's index travels through the DPP row with its
+// row values, all-ones when its row is outside the candidates, which makes
+// the key of an invalid pair all-ones, larger than every valid one.  Key
+// kinds:
+//   KEY_NARROW  32 bits, sums below 2^23 (bounded distances, alpha);
+//   KEY_SAT     32 bits, the sum saturated at KEY_SAT_MAX: exact wherever the
+//               block's least sum lies below it, which the caller checks
+//               (a block at the cap scans again with KEY_WIDE);
+//   KEY_WIDE    64 bits, sums below 2^31.
+// The running (sum, order) minimum is merged once per row-block pair.
+enum { KEY_NARROW = 0, KEY_SAT = 1, KEY_WIDE = 2 };
+#define KEY_SAT_MAX 0x0ffffffeu  // (KEY_SAT_MAX << 4 | 15) stays below all-ones
+template <int KM> struct ScanKeyT { typedef uint32_t T; };
+template <> struct ScanKeyT<KEY_WIDE> { typedef uint64_t T; };
 
-template <int R, int END, bool WIDE> struct KeyStepsOff {
-	typedef typename ScanKeyT<WIDE>::T K;
-	static S2TC_D void run(const int (&X)[16], const int (&Y)[16], uint32_t w, K &kmin)
+// The cap of the KEY_SAT sums: KEY_SAT_MAX, or lower for tests (set by
+// k_set_scan_cap), which sends blocks through the exact rescan.
+static __device__ uint32_t s2tc_scan_cap = KEY_SAT_MAX;
+
+template <int KM> S2TC_D typename ScanKeyT<KM>::T pair_key(int s, uint32_t b, uint32_t cap)
+{
+	if constexpr (KM == KEY_WIDE) {
+		// an invalid partner (all-ones) sets the high word to >= 2^28 - 1 > s >> 28
+		return (((uint64_t)(uint32_t)s << 4) | b) | ((uint64_t)(b >> 4) << 32);
+	} else if constexpr (KM == KEY_SAT) {
+		return (min((uint32_t)s, cap) << 4) | b;
+	} else {
+		return ((uint32_t)s << 4) | b;
+	}
+}
+template <int KM> S2TC_D bool key_valid(typename ScanKeyT<KM>::T k)
+{
+	if constexpr (KM == KEY_WIDE) return (k >> 32) < 8u;
+	else return k != ~0u;
+}
+
+template <int R, int END, int KM> struct KeyStepsOff {
+	typedef typename ScanKeyT<KM>::T K;
+	static S2TC_D void run(const int (&X)[16], const int (&Y)[16], uint32_t w, uint32_t cap, K &kmin)
 	{
 		int s = 0;
 #pragma unroll
@@ -185,22 +214,16 @@ template <int R, int END, bool WIDE> struct KeyStepsOff {
 			s += X[k] < y ? X[k] : y;
 		}
 		const uint32_t b = R ? dpp_u<0x120 + R>(w) : w;
-		if constexpr (WIDE) {
-			// an invalid partner (all-ones) sets the high word to >= 2^28 - 1 > s >> 28
-			const uint64_t key = (((uint64_t)(uint32_t)s << 4) | b) | ((uint64_t)(b >> 4) << 32);
-			kmin = min(kmin, key);
-		} else {
-			kmin = min(kmin, ((uint32_t)s << 4) | b);
-		}
-		KeyStepsOff<R + 1, END, WIDE>::run(X, Y, w, kmin);
+		kmin = min(kmin, pair_key<KM>(s, b, cap));
+		KeyStepsOff<R + 1, END, KM>::run(X, Y, w, cap, kmin);
 	}
 };
-template <int END, bool WIDE> struct KeyStepsOff<END, END, WIDE> {
-	static S2TC_D void run(const int (&)[16], const int (&)[16], uint32_t, typename ScanKeyT<WIDE>::T &) {}
+template <int END, int KM> struct KeyStepsOff<END, END, KM> {
+	static S2TC_D void run(const int (&)[16], const int (&)[16], uint32_t, uint32_t, typename ScanKeyT<KM>::T &) {}
 };
-template <int R, int END, bool WIDE> struct KeyStepsDiag {
-	typedef typename ScanKeyT<WIDE>::T K;
-	static S2TC_D void run(const int (&X)[16], int t, uint32_t w, K &kmin)
+template <int R, int END, int KM> struct KeyStepsDiag {
+	typedef typename ScanKeyT<KM>::T K;
+	static S2TC_D void run(const int (&X)[16], int t, uint32_t w, uint32_t cap, K &kmin)
 	{
 		int s = 0;
 #pragma unroll
@@ -209,16 +232,15 @@ template <int R, int END, bool WIDE> struct KeyStepsDiag {
 			s += X[k] < y ? X[k] : y;
 		}
 		const uint32_t b = dpp_u<0x120 + R>(w);
-		const uint32_t tb = b & 15u, lo = min((uint32_t)t, tb), hi = max((uint32_t)t, tb);
-		K key = ((K)(uint32_t)s << 8) | (K)((lo << 4) | hi);
-		// a partner outside the candidates; R = 8 meets each pair twice, lanes t < 8 keep it
-		key = (b > 15u || (R == 8 && t >= 8)) ? ~(K)0 : key;
+		K key = pair_key<KM>(s, b, cap);
+		// R = 8 meets each pair twice, lanes t < 8 keep it
+		if (R == 8) key = t >= 8 ? ~(K)0 : key;
 		kmin = min(kmin, key);
-		KeyStepsDiag<R + 1, END, WIDE>::run(X, t, w, kmin);
+		KeyStepsDiag<R + 1, END, KM>::run(X, t, w, cap, kmin);
 	}
 };
-template <int END, bool WIDE> struct KeyStepsDiag<END, END, WIDE> {
-	static S2TC_D void run(const int (&)[16], int, uint32_t, typename ScanKeyT<WIDE>::T &) {}
+template <int END, int KM> struct KeyStepsDiag<END, END, KM> {
+	static S2TC_D void run(const int (&)[16], int, uint32_t, uint32_t, typename ScanKeyT<KM>::T &) {}
 };
 
 // MAP: the rows are those of a candidate list without repeated colours,
@@ -226,10 +248,13 @@ template <int END, bool WIDE> struct KeyStepsDiag<END, END, WIDE> {
 // colour's second occurrence (all-ones if none), row sum}; pair orders are
 // formed from the original indices, and each colour that occurs twice also
 // offers its pair (first, second), whose sum is its row sum (min(x, x) = x).
-template <bool WIDE, bool MAP = false>
-S2TC_D uint32_t reduce_rows_key(const int *rows, int t, int m, int pbw, const uint32_t *ul = nullptr)
+// Returns the winning pair order; best: the least sum (saturated at cap for
+// KEY_SAT), row-uniform like the order.
+template <int KM, bool MAP = false>
+S2TC_D uint32_t reduce_rows_key(const int *rows, int t, int m, int pbw, const uint32_t *ul, uint32_t cap,
+                                uint32_t &best)
 {
-	typedef typename ScanKeyT<WIDE>::T K;
+	typedef typename ScanKeyT<KM>::T K;
 	ScanKey st{0xffffffffu, 0xffffffffu, -1};
 	const auto orig = [&](uint32_t a) -> uint32_t { return MAP ? (ul[3 * a] & 0xffffu) : a; };
 	for (int p = 0; p < pbw; ++p) {
@@ -239,26 +264,29 @@ S2TC_D uint32_t reduce_rows_key(const int *rows, int t, int m, int pbw, const ui
 		const bool irow = i < m;
 		{
 			K kmin = ~(K)0;
-			KeyStepsDiag<1, 9, WIDE>::run(X, t, irow ? (uint32_t)t : 0xffffffffu, kmin);
-			const uint32_t lo = 16u * p + (uint32_t)((kmin >> 4) & 15u), hi = 16u * p + (uint32_t)(kmin & 15u);
-			const bool ok = irow && kmin != ~(K)0;
-			merge_key((uint32_t)(kmin >> 8), ok ? (orig(lo) << 16) | orig(hi) : 0u, ok, st);
+			KeyStepsDiag<1, 9, KM>::run(X, t, irow ? (uint32_t)t : 0xffffffffu, cap, kmin);
+			const uint32_t b = (uint32_t)(kmin & 15u), tt = (uint32_t)t;
+			const uint32_t lo = 16u * p + min(tt, b), hi = 16u * p + max(tt, b);
+			const bool ok = irow && key_valid<KM>(kmin);
+			merge_key((uint32_t)(kmin >> 4), ok ? (orig(lo) << 16) | orig(hi) : 0u, ok, st);
 		}
 		for (int q = p + 1; q < pbw; ++q) {
 			int Y[16];
 			load_row(rows, 16 * q + t, Y);
 			K kmin = ~(K)0;
-			KeyStepsOff<0, 16, WIDE>::run(X, Y, 16 * q + t < m ? (uint32_t)t : 0xffffffffu, kmin);
-			const bool ok = irow && (WIDE ? (kmin >> 32) < 8u : kmin != ~(K)0);
+			KeyStepsOff<0, 16, KM>::run(X, Y, 16 * q + t < m ? (uint32_t)t : 0xffffffffu, cap, kmin);
+			const bool ok = irow && key_valid<KM>(kmin);
 			const uint32_t j = 16u * q + (uint32_t)(kmin & 15u);
 			merge_key((uint32_t)(kmin >> 4), ok ? (orig((uint32_t)i) << 16) | orig(j) : 0u, ok, st);
 		}
 		if (MAP && irow) {
 			const uint32_t second = ul[3 * i + 1];
-			merge_key(ul[3 * i + 2], (orig((uint32_t)i) << 16) | second, second != 0xffffffffu, st);
+			const uint32_t rs = KM == KEY_SAT ? min(ul[3 * i + 2], cap) : ul[3 * i + 2];
+			merge_key(rs, (orig((uint32_t)i) << 16) | second, second != 0xffffffffu, st);
 		}
 	}
 	row_min_key(st.hi, st.lo);
+	best = st.hi;
 	return st.lo;
 }
 
@@ -629,13 +657,20 @@ S2TC_D void encode_block_normal(uint32_t tex, int t, int w, int h, int dxt, int 
 	if (cuni) {
 		c0 = c1 = none ? 0u : (uint32_t)vmin;
 	} else {
-		uint32_t cbest;
+		uint32_t cbest, best;
 		if constexpr (DistBound<CD>::value != 0) {
-			if (dd) cbest = reduce_rows_key<false, true>(sm.rows, t, mu, pbu, ul);
-			else cbest = reduce_rows_key<false>(sm.rows, t, m, pbw);
+			if (dd) cbest = reduce_rows_key<KEY_NARROW, true>(sm.rows, t, mu, pbu, ul, 0u, best);
+			else cbest = reduce_rows_key<KEY_NARROW>(sm.rows, t, m, pbw, nullptr, 0u, best);
 		} else if constexpr (IsSymmetric<CD>::value) {
-			if (dd) cbest = reduce_rows_key<true, true>(sm.rows, t, mu, pbu, ul);
-			else cbest = reduce_rows_key<true>(sm.rows, t, m, pbw);
+			// saturated keys first; a block whose least sum reaches the cap
+			// scans again with exact 64-bit keys
+			const uint32_t cap = s2tc_scan_cap;
+			if (dd) cbest = reduce_rows_key<KEY_SAT, true>(sm.rows, t, mu, pbu, ul, cap, best);
+			else cbest = reduce_rows_key<KEY_SAT>(sm.rows, t, m, pbw, nullptr, cap, best);
+			if (best >= cap) {
+				if (dd) cbest = reduce_rows_key<KEY_WIDE, true>(sm.rows, t, mu, pbu, ul, 0u, best);
+				else cbest = reduce_rows_key<KEY_WIDE>(sm.rows, t, m, pbw, nullptr, 0u, best);
+			}
 		} else {
 			cbest = reduce_rows<true>(sm.rows, t, m, pbw);
 		}
@@ -671,7 +706,8 @@ S2TC_D void encode_block_normal(uint32_t tex, int t, int w, int h, int dxt, int 
 	if (dxt == DXT5 && auni) {
 		a0 = a1 = aone;
 	} else if (dxt == DXT5) {
-		const uint32_t abest = reduce_rows_key<false>(sm.rows, t, m, pbw);
+		uint32_t best;
+		const uint32_t abest = reduce_rows_key<KEY_NARROW>(sm.rows, t, m, pbw, nullptr, 0u, best);
 		const int ai = min((int)(abest >> 16), m - 1), aj = min((int)(abest & 0xffffu), m - 1);
 		a0 = (int)(sm.cand[ai] >> 16);
 		a1 = (int)(sm.cand[aj] >> 16);

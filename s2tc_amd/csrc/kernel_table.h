@@ -42,6 +42,7 @@ struct KernelTable {
 	const void *all16_batch_alone[3][3];  // the form for a batch of this one class
 	const void *error[3][2];      // [dxt][raw]: k_colordist_error (total colour distance of an encoding)
 	const void *init;             // per-device table set-up kernel (256 x 256 threads, one int argument), or null
+	const void *set_scan_cap;     // k_set_scan_cap (one workgroup of 64 threads, one uint32_t argument)
 };
 
 extern "C" {

@@ -491,6 +491,14 @@ __global__ __launch_bounds__(256) void k_tables_init(int)
 #endif
 }
 
+// Sets the cap of this kernel object's saturated pair-scan keys on the
+// current device (one thread; the value is clamped to KEY_SAT_MAX).
+template <int CD>
+__global__ __launch_bounds__(64) void k_set_scan_cap(uint32_t cap)
+{
+	if (threadIdx.x == 0) s2tc_scan_cap = min(cap, KEY_SAT_MAX);
+}
+
 }  // namespace s2tc
 
 using namespace s2tc;
@@ -569,6 +577,7 @@ extern "C" void S2TC_CAT(s2tc_kernels_cd, S2TC_CD)(KernelTable *kt)
 #else
 	kt->init = nullptr;
 #endif
+	kt->set_scan_cap = (const void *)k_set_scan_cap<S2TC_CD>;
 	kt->normal_smem_per_cand = (int)(4 * (sizeof(uint32_t) + S2TC_RS * sizeof(int)));
 	kt->normal_smem_fixed = (int)(4 * 16 * sizeof(Dist<S2TC_CD>::F));
 }

@@ -238,4 +238,18 @@ int s2tc_amd_encode_image(const void *d_src, int src_kind, int width, int height
 	return launch_encode(a, (hipStream_t)stream);
 }
 
+int s2tc_amd_debug_set_scan_cap(unsigned cap)
+{
+	if (int r = ensure_device()) return r;
+	for (int cd = 0; cd < 8; ++cd) {
+		const KernelTable &kt = kernel_table(cd);
+		if (!kt.set_scan_cap) continue;
+		void *args[] = {&cap};
+		if (int r = hip_check(hipLaunchKernel(kt.set_scan_cap, dim3(1), dim3(64), args, 0, nullptr),
+		                      "launch k_set_scan_cap"))
+			return r;
+	}
+	return hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+}
+
 }  // extern "C"

@@ -51,3 +51,4 @@ int launch_rgb565(const void *d_src, int srccomps, int w, int h, int alphabits, 
 inline int block_bytes(int dxt) { return dxt == DXT1 ? 8 : 16; }
 
 }  // namespace s2tc
+

@@ -158,3 +158,63 @@ def read_capture(path):
         out.append((int(w), int(h), int(c), np.frombuffer(data, np.uint8, n, pos).copy()))
         pos += n
     return out
+
+
+def corner_blocks(rng, w, h, dxt):
+    """Quantized texels at the corners of the colour cube (each channel 0 or
+    its maximum), opaque: blocks whose candidate pairs leave large
+    distances, so that the least pair sum of a block can exceed 2^28 for the
+    unbounded distances (RGB, YUV, SRGB_MIXED)."""
+    ab = (1 << ALPHABITS[dxt]) - 1
+    q = np.zeros((h, w, 4), np.uint8)
+    for c, top in enumerate((31, 63, 31)):
+        q[..., c] = top * rng.integers(0, 2, (h, w))
+    q[..., 3] = ab
+    return q.reshape(-1)
+
+
+def _mix32(x):
+    x &= 0xffffffff
+    x ^= x >> 16
+    x = (x * 0x7feb352d) & 0xffffffff
+    x ^= x >> 15
+    x = (x * 0x846ca68b) & 0xffffffff
+    x ^= x >> 16
+    return x
+
+
+def prng_block_key(seed, block):
+    """s2tc_prng_block_key of include/s2tc_prng.h."""
+    k = _mix32((seed & 0xffffffff) ^ 0x9E3779B9)
+    k = _mix32(k ^ (seed >> 32))
+    return _mix32(k ^ block)
+
+
+def prng_draw(bkey, k):
+    """s2tc_prng_draw of include/s2tc_prng.h."""
+    return _mix32(bkey + k * 0x9E3779B9) & 0x7FFFFFFF
+
+
+def least_pair_sums(oracle, q, w, h, dxt, cd, nrandom, seed):
+    """Per block of an opaque quantized image: the least sum over candidate
+    pairs i < j of sum_k min(d(c_i, t_k), d(c_j, t_k)), with the candidate
+    list of the random-candidate mode (the block's texels in x-major order,
+    then nrandom colours drawn from their box) and the oracle's distance."""
+    q = np.asarray(q, np.uint8).reshape(h, w, 4)
+    bw, bh = (w + 3) // 4, (h + 3) // 4
+    dpc = 4 if dxt == 2 else 3
+    out = []
+    for by in range(bh):
+        for bx in range(bw):
+            tex = [tuple(int(v) for v in q[4 * by + y, 4 * bx + x, :3])
+                   for x in range(4) for y in range(4) if 4 * bx + x < w and 4 * by + y < h]
+            lo = [min(t[c] for t in tex) for c in range(3)]
+            ln = [max(t[c] for t in tex) - lo[c] + 1 for c in range(3)]
+            key = prng_block_key(seed, by * bw + bx)
+            cand = list(tex)
+            for c in range(nrandom):
+                cand.append(tuple(lo[z] + prng_draw(key, c * dpc + z) % ln[z] for z in range(3)))
+            d = np.array([[oracle.color_dist(cd, ci, tk) for tk in tex] for ci in cand], np.int64)
+            best = min(int(np.minimum(d[i], d[i + 1:]).sum(1).min()) for i in range(len(cand) - 1))
+            out.append(best)
+    return out

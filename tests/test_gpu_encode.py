@@ -88,6 +88,45 @@ def test_random_candidates_small_boxes(gpu, oracle, cd):
             _compare(gpu, oracle, q, w, h, dxt, cd, nr, refine, 9 + cd, "smallbox")
 
 
+def _opaque(q, dxt):
+    q = q.reshape(-1, 4).copy()
+    q[:, 3] = (1 << imgutil.ALPHABITS[dxt]) - 1
+    return q.reshape(-1)
+
+
+@pytest.mark.parametrize("cd", [0, 1, 3])
+def test_random_candidates_saturated_keys(gpu, oracle, cd):
+    """The pair scan of the unbounded distances (RGB, YUV, SRGB_MIXED) keys
+    its sums saturated at a cap (2^28 - 2; no block of 565 colours with 64
+    random candidates has come near it), and a block whose least pair sum
+    reaches the cap scans again with exact 64-bit keys.  With the cap set to
+    the median least sum of the image (computed here on the CPU), half of the
+    blocks take each path, through the plain scan (wide colour boxes) and the
+    scan over the list without repeated colours (small boxes)."""
+    import ctypes
+    L = gpu.lib()
+    L.s2tc_amd_debug_set_scan_cap.argtypes = [ctypes.c_uint]
+    rng = np.random.default_rng(500 + cd)
+    w, h, nr, seed = 32, 16, 64, 13
+    try:
+        for kind, dxt in itertools.product(["random", "smallbox", "corners"], range(3)):
+            if kind == "smallbox":
+                q = _opaque(imgutil.small_box_blocks(rng, w, h, dxt), dxt)
+            elif kind == "corners":
+                q = imgutil.corner_blocks(rng, w, h, dxt)
+            else:
+                q = _opaque(imgutil.random_quantized(rng, w, h, dxt, "random"), dxt)
+            sums = sorted(imgutil.least_pair_sums(oracle, q, w, h, dxt, cd, nr, seed))
+            cap = sums[len(sums) // 2]
+            assert sums[0] < cap <= sums[-1], f"{kind}: least sums do not straddle the cap"
+            _say(f"scan cap {cap} ({kind}, dxt={dxt}, cd={cd})")
+            gpu.check(L.s2tc_amd_debug_set_scan_cap(cap))
+            for refine in range(3):
+                _compare(gpu, oracle, q, w, h, dxt, cd, nr, refine, seed, f"cap-{kind}")
+    finally:
+        gpu.check(L.s2tc_amd_debug_set_scan_cap(0xffffffff))
+
+
 def test_raw_rgba8_source(gpu, oracle):
     """Fused DITHER_NONE quantization on load equals rgb565_image + encode."""
     rng = np.random.default_rng(5)
