@@ -41,11 +41,11 @@ extern "C" {
 
 /* Random candidates per block.  Up to S2TC_LDS_NRANDOM the candidate list
  * (16 + nrandom colours, rounded up to 16, 4 bytes each), its distance rows
- * (64 bytes each) and the features of the first 16 candidates of four
- * blocks fit the 64 KiB of LDS a workgroup can allocate (224 candidates
- * take 61,696 bytes, 240 would take 66,048).  Above it the same encoder
- * keeps them in device scratch memory (m * 68 bytes per block in flight,
- * allocated stream-ordered per call, or with a batch).  S2TC_MAX_NRANDOM
+ * from the 17th on (64 bytes each; the first 16 stay in registers) and the
+ * features of the first 16 candidates of four blocks fit the 64 KiB of LDS
+ * a workgroup can allocate (224 candidates take 57,600 bytes).  Above it
+ * the same encoder keeps them in device scratch memory (under m * 68 bytes
+ * per block in flight, allocated stream-ordered per call, or with a batch).  S2TC_MAX_NRANDOM
  * bounds the candidate indices to 15 bits (16 + nrandom rounded up to 16
  * stays at or below 32768); larger values are S2TC_E_RANGE. */
 #define S2TC_LDS_NRANDOM 208
@@ -87,8 +87,8 @@ S2TC_API int s2tc_amd_encode_image(const void *d_src, int src_kind, int width, i
  * memory (code under stream capture, tight serving loops, hosts that forbid
  * hipMalloc): s2tc_amd_encode_image with the same arguments and output,
  * calling no hipMalloc/hipFree or allocating routine.  Random-candidate
- * encodes above S2TC_LDS_NRANDOM take their group memory (4 * 68 bytes per
- * candidate slot per workgroup) from a static device array of
+ * encodes above S2TC_LDS_NRANDOM take their group memory (under 4 * 68
+ * bytes per candidate slot per workgroup) from a static device array of
  * S2TC_NOALLOC_SCRATCH bytes that each GPU holds until the process ends,
  * with as many workgroups as fit in it.  Calls that overlap on one GPU share
  * that array and race: serialize them (one stream, or one host thread). */

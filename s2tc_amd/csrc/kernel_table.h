@@ -32,7 +32,8 @@ struct KernelTable {
 	const void *fast_int[3][2][3];  // [dxt][raw][refine]: k_fast_interior (full blocks, 16-byte rows), null for NORMALMAP
 	const void *normal[2];        // [raw]: k_normal<CD, raw>
 	const void *normal_wide[2];   // [raw]: k_normal_wide<CD, raw> (group memory in global scratch)
-	int normal_smem_per_cand;     // LDS bytes per candidate slot for a 4-block workgroup
+	int normal_smem_cand;         // LDS bytes per candidate slot of a 4-block workgroup (candidate words)
+	int normal_smem_row;          // LDS bytes per stored distance row of a 4-block workgroup
 	int normal_smem_fixed;        // plus these bytes (features of the 16 texel candidates of 4 blocks)
 	const void *fast_batch[3][3]; // [dxt][refine]: persistent quick-mode kernel over a job list
 	const void *normal_batch;     // persistent MODE_NORMAL kernel over a job list
@@ -44,6 +45,15 @@ struct KernelTable {
 	const void *init;             // per-device table set-up kernel (256 x 256 threads, one int argument), or null
 	const void *set_scan_cap;     // k_set_scan_cap (one workgroup of 64 threads, one uint32_t argument)
 };
+
+// Group memory of a four-block workgroup of the random-candidate kernels
+// with pbw row blocks: 16 pbw candidate slots, the rows from 16 on (row
+// block 0 stays in registers) and the texel features.
+inline size_t normal_group_smem(const KernelTable &kt, int pbw)
+{
+	return (size_t)kt.normal_smem_cand * 16 * (size_t)pbw + (size_t)kt.normal_smem_row * 16 * (size_t)(pbw - 1) +
+	       (size_t)kt.normal_smem_fixed;
+}
 
 extern "C" {
 void s2tc_kernels_cd0(KernelTable *);

@@ -578,6 +578,7 @@ extern "C" void S2TC_CAT(s2tc_kernels_cd, S2TC_CD)(KernelTable *kt)
 	kt->init = nullptr;
 #endif
 	kt->set_scan_cap = (const void *)k_set_scan_cap<S2TC_CD>;
-	kt->normal_smem_per_cand = (int)(4 * (sizeof(uint32_t) + S2TC_RS * sizeof(int)));
+	kt->normal_smem_cand = (int)(4 * sizeof(uint32_t));
+	kt->normal_smem_row = (int)(4 * S2TC_RS * sizeof(int));
 	kt->normal_smem_fixed = (int)(4 * 16 * sizeof(Dist<S2TC_CD>::F));
 }

@@ -113,7 +113,7 @@ int plan_classes(const s2tc_amd_job *jobs, int njobs, std::vector<ClassPlan> &cl
 		                      : kt.fast_batch[c.dxt][c.refine];
 		if (!c.kernel) return set_error(S2TC_E_ARG, "no batch kernel for distance %d", c.cd);
 		c.ntiles = (int)((c.nblocks + c.tile - 1) / c.tile);
-		c.smem = c.normal && !c.wide ? (size_t)kt.normal_smem_per_cand * 16 * c.pbw_max + (size_t)kt.normal_smem_fixed : 0;
+		c.smem = c.normal && !c.wide ? normal_group_smem(kt, c.pbw_max) : 0;
 		if (c.wide) wide_launch_shape(kt, c.pbw_max, c.ntiles, device_cu_count(), c.wide_grid, c.scratch);
 		tile_ranges(c.jobs, c.nblocks, c.tile, c.ntiles, c.tiles);
 		if (c.all16) {
@@ -124,7 +124,7 @@ int plan_classes(const s2tc_amd_job *jobs, int njobs, std::vector<ClassPlan> &cl
 				// the random-candidate kernel runs nrandom-0 jobs as they are
 				c.forms[FORM_ROW16] = kt.normal_batch;
 				c.ntiles4 = (int)((c.nblocks + 3) / 4);
-				c.smem4 = (size_t)kt.normal_smem_per_cand * 16 + (size_t)kt.normal_smem_fixed;
+				c.smem4 = normal_group_smem(kt, 1);
 				tile_ranges(c.jobs, c.nblocks, 4, c.ntiles4, c.tiles4);
 			}
 		}
@@ -183,7 +183,7 @@ void plan_set_scratch(std::vector<ClassPlan> &classes, uint8_t *dev)
 
 void wide_launch_shape(const KernelTable &kt, int pbw, long long ntiles, int ncu, int &grid, size_t &bytes)
 {
-	const size_t wg = (size_t)kt.normal_smem_per_cand * 16 * (size_t)pbw + (size_t)kt.normal_smem_fixed;
+	const size_t wg = normal_group_smem(kt, pbw);
 	const long long cap = std::max(1LL, (long long)(((size_t)1 << 30) / wg));
 	grid = (int)std::max(1LL, std::min(ntiles, std::min(4LL * ncu, cap)));
 	bytes = wg * (size_t)grid;

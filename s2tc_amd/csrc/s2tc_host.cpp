@@ -159,7 +159,7 @@ int launch_encode(const EncodeArgs &a, hipStream_t stream, void *ext_scratch, si
 	unsigned long long seed = a.seed;
 	unsigned block_base = a.block_index_base;
 	if (nrandom <= S2TC_LDS_NRANDOM) {
-		size_t smem = (size_t)kt.normal_smem_per_cand * 16 * pbw + (size_t)kt.normal_smem_fixed;
+		size_t smem = normal_group_smem(kt, pbw);
 		void *args[] = {(void *)&src, (void *)&width, (void *)&height, (void *)&dst, (void *)&dxt,
 		                (void *)&nrandom, (void *)&refine, (void *)&seed, (void *)&pbw, (void *)&block_base};
 		dim3 grid((unsigned)((nb + 3) / 4)), block(64);
