@@ -12,7 +12,13 @@ Printed (one JSON line): Mtexels/s of the batched launch (value), the same
 jobs as plain per-image launches, the roofline of the dominant kernel class,
 and the compiled reference's CPU throughput on a sample of the same jobs.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload batch|tiles4096|normal8192|sinus256]
+  python bench.py [--gpus 1] [--steps K] [--warmup W]
+                  [--workload batch|servepool|tiles4096|normal8192|sinus256]
+
+servepool: the serving leg's pool of 256 textures (serving mix: 70% quick
+mode, 30% nrandom 64 + REFINE_LOOP, full mip chains) as one batch - the
+capacity that the serving loop schedules - checked against the reference's
+per-texture digests.
 """
 import argparse
 import ctypes
@@ -504,15 +510,23 @@ def serve_leg(s, rates, duration, npool=256, seed=4):
 
 
 def verify_batch_output(imgs, workload):
-    """SHA-256 over the per-image block digests of the batch output, against
-    the compiled reference's digest of the same jobs (tests/golden/
+    """SHA-256 over the per-image block digests of the batch output (batch),
+    or per texture over its mip levels' blocks (servepool), against the
+    compiled reference's digests of the same jobs (tests/golden/
     digests.json).  True / False, or None where no digest exists."""
     import hashlib
     digests = json.load(open(os.path.join(ROOT, "tests", "golden", "digests.json")))["workloads"]
-    exp = digests.get(workload) if workload == "batch" else None
+    exp = digests.get({"batch": "batch", "servepool": "serve_pool"}.get(workload, ""))
     if not exp or len(imgs.items) != exp["jobs"]:
         return None
     host = imgs.dst.to_host()
+    if workload == "servepool":
+        f, ok = 0, True
+        for n, want in zip(imgs.levels_per_texture, exp["texture_sha256"]):
+            got = hashlib.sha256(b"".join(imgs.output(f + q, host).tobytes() for q in range(n))).hexdigest()
+            ok = ok and got == want
+            f += n
+        return ok and f == len(imgs.items)
     hashes = b"".join(hashlib.sha256(imgs.output(i, host).tobytes()).digest() for i in range(len(imgs.items)))
     return hashlib.sha256(hashes).hexdigest() == exp["sha256_of_job_sha256s"]
 
@@ -613,14 +627,16 @@ def main():
 
     t0 = time.time()
     mip_leg = None
-    if a.workload == "batch":
+    if a.workload in ("batch", "servepool"):
         from s2tc_amd import workloads
-        textures = workloads.batch_textures(3, 2000)
-        log(f"workload batch: {len(textures)} base levels generated in {time.time() - t0:.1f} s")
+        textures = workloads.batch_textures(3, 2000) if a.workload == "batch" else \
+            workloads.batch_textures(workloads.SERVE_SEED, workloads.SERVE_POOL, serve_mix=True)
+        log(f"workload {a.workload}: {len(textures)} base levels generated in {time.time() - t0:.1f} s")
         t1 = time.perf_counter()
         imgs = s.DeviceImages.from_textures(textures)
         mip_leg = dict(textures=len(textures), uploaded_bytes=imgs.uploaded_bytes,
                        upload_and_chain_s=round(time.perf_counter() - t1, 3))
+        imgs.levels_per_texture = [max(t["width"], t["height"]).bit_length() for t in textures]
         del textures
         items = imgs.items
     else:
@@ -749,9 +765,11 @@ def main():
         "value": round(value, 1), "unit": "Mtexels/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
         "ms_per_step": round(dt * 1e3, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "u8", "data": "synthetic",
-        "config": {"workload": a.workload if a.workload != "batch" else
-                   "batch: 2000 RGBA8 textures 4..2048 per side + full mip chains (16467 images), "
-                   "DXT1/3/5 30/30/40%, 8 distance modes, quick selection, REFINE_ALWAYS, DITHER_NONE",
+        "config": {"workload": {"batch": "batch: 2000 RGBA8 textures 4..2048 per side + full mip chains (16467 images), "
+                                "DXT1/3/5 30/30/40%, 8 distance modes, quick selection, REFINE_ALWAYS, DITHER_NONE",
+                                "servepool": "servepool: the serving pool as one batch, 256 RGBA8 textures of the "
+                                "serving mix (70% quick, 30% nrandom 64 + REFINE_LOOP) + full mip chains"
+                                }.get(a.workload, a.workload),
                    "images": len(items), "texels": texels, "schedule": {"wg_per_cu": sched[0], "streams": sched[1]}},
         "bit_exact": bit_exact,
         "plain_per_texture": plain,
