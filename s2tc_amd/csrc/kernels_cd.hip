@@ -403,6 +403,60 @@ S2TC_D void normal_batch_tile(const JobDesc *__restrict__ jobs, const TileJobs *
 	}
 }
 
+// One block's inputs, fetched a tile ahead of its encode so that the chain of
+// dependent loads (tile -> job -> texels) overlaps the previous tile's work.
+struct BlockIn {
+	uint32_t tex;  // the texel word at a clamped address (every lane loads; see encode_fetched)
+	int b, w, h, dxt, nrandom, refine, raw;
+	uint32_t bkey;
+	uint8_t *dst;
+	bool have;
+};
+
+S2TC_D BlockIn fetch_block(const JobDesc *__restrict__ jobs, TileJobs tj, int tile, long long nblocks, int g, int t)
+{
+	BlockIn in;
+	long long gb = (long long)tile * 4 + g;
+	in.have = gb < nblocks;
+	if (!in.have) gb = nblocks - 1;
+	const JobDesc &j = jobs[find_job(jobs, tj.lo, tj.hi, gb)];
+	in.b = (int)(gb - j.block0);
+	const int bw = (j.width + 3) >> 2;
+	int by, bx;
+	div_blocks(in.b, bw, j.bw_magic, by, bx);
+	in.w = min(4, j.width - 4 * bx);
+	in.h = min(4, j.height - 4 * by);
+	// an unconditional load (the address clamped into the block's valid
+	// texels), so that nothing waits for it before the texel is used
+	const int x = min(t >> 2, in.w - 1), y = min(t & 3, in.h - 1);
+	in.tex = j.src[(size_t)(4 * by + y) * (size_t)j.width + 4 * bx + x];
+	in.dxt = j.dxt;
+	in.nrandom = j.nrandom;
+	in.refine = j.refine;
+	in.raw = j.raw;
+	in.bkey = s2tc_prng_block_key((uint64_t)j.seed, (uint32_t)in.b);
+	in.dst = j.dst;
+	return in;
+}
+
+template <int CD> S2TC_D void encode_fetched(const BlockIn &in, const GroupSmem<CD> &sm, int t)
+{
+	const int x = t >> 2, y = t & 3;
+	uint32_t tex = 0;
+	if (x < in.w && y < in.h) tex = in.raw ? quant_none_rt(in.tex, in.dxt) : in.tex;
+	const int pbw = (16 + (in.nrandom > 0 ? in.nrandom : 0) + 15) / 16;
+	BlockOut o;
+	encode_block_normal<CD, true>(tex, t, in.w, in.h, in.dxt, in.nrandom, in.refine, in.bkey, pbw, sm, o);
+	if (in.have && t == 0) {
+		if (in.dxt == DXT1) store_block<DXT1>(in.dst, (size_t)in.b, o);
+		else store_block<DXT5>(in.dst, (size_t)in.b, o);
+	}
+}
+
+// Software-pipelined over the workgroup's tiles: while tile T encodes, the
+// job and texels of tile T + G are in flight and the tile record of T + 2G
+// (G = grid size; indices past the end are clamped to the last tile, whose
+// inputs are then fetched and never used).
 template <int CD>
 __global__ __launch_bounds__(64) void k_normal_batch(const JobDesc *__restrict__ jobs, const TileJobs *__restrict__ tile_jobs,
                                                      long long nblocks, int ntiles, int pbw_max)
@@ -411,9 +465,22 @@ __global__ __launch_bounds__(64) void k_normal_batch(const JobDesc *__restrict__
 	const int lane = (int)threadIdx.x;
 	const int g = lane >> 4, t = lane & 15;
 	const GroupSmem<CD> sm = group_smem<CD>(smem, g, 16 * pbw_max);
-	for (int tile = xcd_local_id((int)blockIdx.x, (int)gridDim.x); tile < ntiles; tile += (int)gridDim.x) {
-		normal_batch_tile<CD, true>(jobs, tile_jobs, nblocks, sm, tile, g, t);
+	const int G = (int)gridDim.x;
+	int tile = xcd_local_id((int)blockIdx.x, G);
+	if (tile >= ntiles) return;
+	const auto tile_at = [&](int tl) {
+		int i = min(tl, ntiles - 1);
+		asm volatile("" : "+v"(i));  // a vector load: a scalar one would join the encoder's LDS waits
+		return tile_jobs[i];
+	};
+	TileJobs tjn = tile_at(tile + G);
+	BlockIn cur = fetch_block(jobs, tile_at(tile), tile, nblocks, g, t);
+	for (; tile < ntiles; tile += G) {
+		const BlockIn nxt = fetch_block(jobs, tjn, min(tile + G, ntiles - 1), nblocks, g, t);
+		tjn = tile_at(tile + 2 * G);
+		encode_fetched<CD>(cur, sm, t);
 		__syncthreads();
+		cur = nxt;
 	}
 }
 
