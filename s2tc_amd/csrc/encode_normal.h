@@ -26,6 +26,7 @@
 #pragma once
 
 #include "encode_fast.h"
+#include "kernel_table.h"  // S2TC_REG_RB
 
 namespace s2tc {
 
@@ -121,12 +122,17 @@ template <int END, bool DIAG, int PASS, bool NEG> struct ScanSteps<END, END, DIA
 	static S2TC_D void run(const int (&)[16], const int (&)[16], int, int, int, int, int, ScanKey &) {}
 };
 
-// Row r >= 16 of a block's distance matrix (the row memory starts at row 16:
-// row block 0 is only ever the first block of a pair and stays in the
-// registers of its lanes, X0 below).
+// The first S2TC_REG_RB row blocks of a block's distance matrix stay in the
+// registers of their lanes (RegRows); the row memory holds the rows from
+// 16 S2TC_REG_RB on.  Fewer rows in LDS let more workgroups share a CU.
+struct RegRows {
+	int r[S2TC_REG_RB][16];
+};
+
+// Row r >= 16 S2TC_REG_RB from the row memory.
 S2TC_D void load_row(const int *rows, int r, int (&X)[16])
 {
-	const int4 *p = (const int4 *)(rows + (r - 16) * S2TC_RS);
+	const int4 *p = (const int4 *)(rows + (r - 16 * S2TC_REG_RB) * S2TC_RS);
 #pragma unroll
 	for (int k = 0; k < 4; ++k) {
 		int4 v = p[k];
@@ -137,26 +143,46 @@ S2TC_D void load_row(const int *rows, int r, int (&X)[16])
 	}
 }
 
+// Lane t's row of row block p (row 16p + t), wherever it lives.
+S2TC_D void get_rows(const int *rows, const RegRows &R, int p, int t, int (&X)[16])
+{
+	if (p >= S2TC_REG_RB) {
+		load_row(rows, 16 * p + t, X);
+		return;
+	}
+#pragma unroll
+	for (int z = 0; z < S2TC_REG_RB; ++z)
+		if (p == z) {
+#pragma unroll
+			for (int k = 0; k < 16; ++k) X[k] = R.r[z][k];
+		}
+}
+S2TC_D void put_rows(int *rows, RegRows &R, int p, int t, const int (&dv)[16])
+{
+	if (p >= S2TC_REG_RB) {
+		int4 *rp = (int4 *)(rows + (16 * p + t - 16 * S2TC_REG_RB) * S2TC_RS);
+#pragma unroll
+		for (int k = 0; k < 4; ++k) rp[k] = make_int4(dv[4 * k], dv[4 * k + 1], dv[4 * k + 2], dv[4 * k + 3]);
+		return;
+	}
+#pragma unroll
+	for (int z = 0; z < S2TC_REG_RB; ++z)
+		if (p == z) {
+#pragma unroll
+			for (int k = 0; k < 16; ++k) R.r[z][k] = dv[k];
+		}
+}
+
 // Scan over all pairs i < j < m of the rows in LDS; pbw row blocks (uniform).
 template <int PASS, bool NEG>
-S2TC_D void pair_scan(const int *rows, const int (&X0)[16], int t, int m, int pbw, int qthr, ScanKey &st)
+S2TC_D void pair_scan(const int *rows, const RegRows &R, int t, int m, int pbw, int qthr, ScanKey &st)
 {
 	for (int p = 0; p < pbw; ++p) {
 		int X[16];
-		if (p == 0) {
-#pragma unroll
-			for (int k = 0; k < 16; ++k) X[k] = X0[k];
-		} else {
-			load_row(rows, 16 * p + t, X);
-		}
+		get_rows(rows, R, p, t, X);
 		for (int q = p; q < pbw; ++q) {
 			int Y[16];
-			if (q == 0) {
-#pragma unroll
-				for (int k = 0; k < 16; ++k) Y[k] = X0[k];
-			} else {
-				load_row(rows, 16 * q + t, Y);
-			}
+			get_rows(rows, R, q, t, Y);
 			if (q == p)
 				ScanSteps<1, 9, true, PASS, NEG>::run(X, Y, t, p, q, m, qthr, st);
 			else
@@ -264,7 +290,7 @@ template <int END, int KM> struct KeyStepsDiag<END, END, KM> {
 // Returns the winning pair order; best: the least sum (saturated at cap for
 // KEY_SAT), row-uniform like the order.
 template <int KM, bool MAP = false>
-S2TC_D uint32_t reduce_rows_key(const int *rows, const int (&X0)[16], int t, int m, int pbw, const uint32_t *ul,
+S2TC_D uint32_t reduce_rows_key(const int *rows, const RegRows &R, int t, int m, int pbw, const uint32_t *ul,
                                 uint32_t cap, uint32_t &best)
 {
 	typedef typename ScanKeyT<KM>::T K;
@@ -272,12 +298,7 @@ S2TC_D uint32_t reduce_rows_key(const int *rows, const int (&X0)[16], int t, int
 	const auto orig = [&](uint32_t a) -> uint32_t { return MAP ? (ul[3 * a] & 0xffffu) : a; };
 	for (int p = 0; p < pbw; ++p) {
 		int X[16];
-		if (p == 0) {
-#pragma unroll
-			for (int k = 0; k < 16; ++k) X[k] = X0[k];
-		} else {
-			load_row(rows, 16 * p + t, X);
-		}
+		get_rows(rows, R, p, t, X);
 		const int i = 16 * p + t;
 		const bool irow = i < m;
 		{
@@ -290,7 +311,7 @@ S2TC_D uint32_t reduce_rows_key(const int *rows, const int (&X0)[16], int t, int
 		}
 		for (int q = p + 1; q < pbw; ++q) {
 			int Y[16];
-			load_row(rows, 16 * q + t, Y);
+			get_rows(rows, R, q, t, Y);
 			K kmin = ~(K)0;
 			KeyStepsOff<0, 16, KM>::run(X, Y, 16 * q + t < m ? (uint32_t)t : 0xffffffffu, cap, kmin);
 			const bool ok = irow && key_valid<KM>(kmin);
@@ -312,15 +333,15 @@ S2TC_D uint32_t reduce_rows_key(const int *rows, const int (&X0)[16], int t, int
 // winning pair order i << 16 | j (group-uniform).  Candidate indices stay
 // below 2^15 (S2TC_MAX_NRANDOM), so every order is a positive int.
 template <bool NEG>
-S2TC_D uint32_t reduce_rows(const int *rows, const int (&X0)[16], int t, int m, int pbw)
+S2TC_D uint32_t reduce_rows(const int *rows, const RegRows &R, int t, int m, int pbw)
 {
 	ScanKey st{0xffffffffu, 0xffffffffu, -1};
-	pair_scan<1, NEG>(rows, X0, t, m, pbw, -1, st);
+	pair_scan<1, NEG>(rows, R, t, m, pbw, -1, st);
 	if (NEG) {
 		const int q = row_max(st.qneg);
 		if (q >= 0) {
 			st.hi = st.lo = 0xffffffffu;
-			pair_scan<2, NEG>(rows, X0, t, m, pbw, q, st);
+			pair_scan<2, NEG>(rows, R, t, m, pbw, q, st);
 			row_min_key(st.hi, st.lo);
 			return st.lo == 0xffffffffu ? (uint32_t)q : st.lo;
 		}
@@ -329,7 +350,8 @@ S2TC_D uint32_t reduce_rows(const int *rows, const int (&X0)[16], int t, int m, 
 	return st.lo;
 }
 
-// Per-group LDS: rows 16 .. mc - 1 (row block 0 stays in registers), the
+// Per-group LDS: rows 16 S2TC_REG_RB .. mc - 1 (the first row blocks stay in
+// registers), the
 // features of the first 16 candidates (the columns of every row), candidates
 // (565 | alpha << 16).  A row's own candidate feature is computed by its
 // lane.  normal_group_smem (kernel_table.h) gives the host the same size for
@@ -342,14 +364,14 @@ template <int CD> struct GroupSmem {
 
 template <int CD> S2TC_D size_t group_smem_bytes(int mc)
 {
-	return (size_t)mc * sizeof(uint32_t) + (size_t)(mc - 16) * S2TC_RS * sizeof(int) +
+	return (size_t)mc * sizeof(uint32_t) + (size_t)max(mc - 16 * S2TC_REG_RB, 0) * S2TC_RS * sizeof(int) +
 	       16 * sizeof(typename Dist<CD>::F);
 }
 
 template <int CD> S2TC_D GroupSmem<CD> group_smem(uint8_t *base, int g, int mc)
 {
 	uint8_t *p = base + (size_t)g * group_smem_bytes<CD>(mc);
-	const size_t rb = (size_t)(mc - 16) * S2TC_RS * sizeof(int);
+	const size_t rb = (size_t)max(mc - 16 * S2TC_REG_RB, 0) * S2TC_RS * sizeof(int);
 	GroupSmem<CD> s;
 	s.rows = (int *)p;
 	s.feat = (typename Dist<CD>::F *)(p + rb);
@@ -580,8 +602,9 @@ S2TC_D void encode_block_normal(uint32_t tex, int t, int w, int h, int dxt, int 
 	}
 	__syncthreads();
 
-	// Repeated colours (DEDUP).  In this block's row memory (16 (pbw - 1) rows
-	// of S2TC_RS ints): T1[V] and T2[V] over the colour box (V <= 256 cells) take
+	// Repeated colours (DEDUP).  In this block's row memory (16 (pbw -
+	// S2TC_REG_RB) rows of S2TC_RS ints): T1[V] and T2[V] over the colour box
+	// (V <= 256 cells) take
 	// the first and second candidate index of each colour by atomic minimum,
 	// then the list of first occurrences goes to the last 3 m ints, which the
 	// compacted rows (written from the top) must not reach.  Every lane of the
@@ -590,7 +613,7 @@ S2TC_D void encode_block_normal(uint32_t tex, int t, int w, int h, int dxt, int 
 	bool dd = false;
 	uint32_t *ul = nullptr;
 	if constexpr (DEDUP && PlainScan<CD>::value) {
-		const int cap = 16 * (pbw - 1) * S2TC_RS;
+		const int cap = 16 * (pbw - S2TC_REG_RB) * S2TC_RS;
 		const int V = lr * lg * lb;
 		dd = !cuni && nrandom > 0 && pbw >= 2 && V <= 256 && 2 * V + 3 * m <= cap;
 		int *T1 = sm.rows, *T2 = sm.rows + V;
@@ -633,9 +656,9 @@ S2TC_D void encode_block_normal(uint32_t tex, int t, int w, int h, int dxt, int 
 			}
 			mu = base;
 			pbu = (mu + 15) >> 4;
-			// the compacted rows (16 (pbu - 1) of them are stored) must stay
-			// clear of the list, and must save work
-			if (pbu >= pbw || 16 * (pbu - 1) * S2TC_RS + 3 * m > cap) {
+			// the compacted rows (16 (pbu - S2TC_REG_RB) of them are stored)
+			// must stay clear of the list, and must save work
+			if (pbu >= pbw || 16 * max(pbu - S2TC_REG_RB, 0) * S2TC_RS + 3 * m > cap) {
 				dd = false;
 				mu = m;
 				pbu = pbw;
@@ -650,8 +673,8 @@ S2TC_D void encode_block_normal(uint32_t tex, int t, int w, int h, int dxt, int 
 
 	// dists[i][k] rows (:373-392): square part dist(c[min], c[max]), box part
 	// dist(c[i], c[k]); with DEDUP the rows of the list of first occurrences,
-	// each with its row sum.  Row block 0 goes to X0.
-	int X0[16];
+	// each with its row sum.
+	RegRows R;
 	for (int p = 0; p < (cuni ? 0 : pbu); ++p) {
 		const int i = 16 * p + t;
 		const bool irow = i < mu;
@@ -665,14 +688,7 @@ S2TC_D void encode_block_normal(uint32_t tex, int t, int w, int h, int dxt, int 
 			else d = (i < n && k < i) ? D::d(fk, fi) : D::d(fi, fk);
 			dv[k] = (irow && k < n) ? d : 0;
 		}
-		if (p == 0) {
-#pragma unroll
-			for (int k = 0; k < 16; ++k) X0[k] = dv[k];
-		} else {
-			int4 *rp = (int4 *)(sm.rows + (i - 16) * S2TC_RS);
-#pragma unroll
-			for (int k = 0; k < 4; ++k) rp[k] = make_int4(dv[4 * k], dv[4 * k + 1], dv[4 * k + 2], dv[4 * k + 3]);
-		}
+		put_rows(sm.rows, R, p, t, dv);
 		if (dd && irow) {
 			uint32_t rs = 0;
 #pragma unroll
@@ -687,20 +703,20 @@ S2TC_D void encode_block_normal(uint32_t tex, int t, int w, int h, int dxt, int 
 	} else {
 		uint32_t cbest, best;
 		if constexpr (DistBound<CD>::value != 0) {
-			if (dd) cbest = reduce_rows_key<KEY_NARROW, true>(sm.rows, X0, t, mu, pbu, ul, 0u, best);
-			else cbest = reduce_rows_key<KEY_NARROW>(sm.rows, X0, t, m, pbw, nullptr, 0u, best);
+			if (dd) cbest = reduce_rows_key<KEY_NARROW, true>(sm.rows, R, t, mu, pbu, ul, 0u, best);
+			else cbest = reduce_rows_key<KEY_NARROW>(sm.rows, R, t, m, pbw, nullptr, 0u, best);
 		} else if constexpr (IsSymmetric<CD>::value) {
 			// saturated keys first; a block whose least sum reaches the cap
 			// scans again with exact 64-bit keys
 			const uint32_t cap = s2tc_scan_cap;
-			if (dd) cbest = reduce_rows_key<KEY_SAT, true>(sm.rows, X0, t, mu, pbu, ul, cap, best);
-			else cbest = reduce_rows_key<KEY_SAT>(sm.rows, X0, t, m, pbw, nullptr, cap, best);
+			if (dd) cbest = reduce_rows_key<KEY_SAT, true>(sm.rows, R, t, mu, pbu, ul, cap, best);
+			else cbest = reduce_rows_key<KEY_SAT>(sm.rows, R, t, m, pbw, nullptr, cap, best);
 			if (best >= cap) {
-				if (dd) cbest = reduce_rows_key<KEY_WIDE, true>(sm.rows, X0, t, mu, pbu, ul, 0u, best);
-				else cbest = reduce_rows_key<KEY_WIDE>(sm.rows, X0, t, m, pbw, nullptr, 0u, best);
+				if (dd) cbest = reduce_rows_key<KEY_WIDE, true>(sm.rows, R, t, mu, pbu, ul, 0u, best);
+				else cbest = reduce_rows_key<KEY_WIDE>(sm.rows, R, t, m, pbw, nullptr, 0u, best);
 			}
 		} else {
-			cbest = reduce_rows<true>(sm.rows, X0, t, m, pbw);
+			cbest = reduce_rows<true>(sm.rows, R, t, m, pbw);
 		}
 		const int ci = min((int)(cbest >> 16), m - 1), cj = min((int)(cbest & 0xffffu), m - 1);
 		c0 = sm.cand[ci] & 0xffffu;
@@ -725,14 +741,7 @@ S2TC_D void encode_block_normal(uint32_t tex, int t, int w, int h, int dxt, int 
 				const int d = adist(ai, ak);
 				dv[k] = (irow && k < n) ? (d < fk ? d : fk) : 0;
 			}
-			if (p == 0) {
-#pragma unroll
-				for (int k = 0; k < 16; ++k) X0[k] = dv[k];
-			} else {
-				int4 *rp = (int4 *)(sm.rows + (i - 16) * S2TC_RS);
-#pragma unroll
-				for (int k = 0; k < 4; ++k) rp[k] = make_int4(dv[4 * k], dv[4 * k + 1], dv[4 * k + 2], dv[4 * k + 3]);
-			}
+			put_rows(sm.rows, R, p, t, dv);
 		}
 	}
 	__syncthreads();
@@ -740,7 +749,7 @@ S2TC_D void encode_block_normal(uint32_t tex, int t, int w, int h, int dxt, int 
 		a0 = a1 = aone;
 	} else if (dxt == DXT5) {
 		uint32_t best;
-		const uint32_t abest = reduce_rows_key<KEY_NARROW>(sm.rows, X0, t, m, pbw, nullptr, 0u, best);
+		const uint32_t abest = reduce_rows_key<KEY_NARROW>(sm.rows, R, t, m, pbw, nullptr, 0u, best);
 		const int ai = min((int)(abest >> 16), m - 1), aj = min((int)(abest & 0xffffu), m - 1);
 		a0 = (int)(sm.cand[ai] >> 16);
 		a1 = (int)(sm.cand[aj] >> 16);

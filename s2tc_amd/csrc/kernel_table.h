@@ -1,6 +1,7 @@
 // kernel_table.h - per-distance-mode table of encoder kernels (host side).
 #pragma once
 
+#include <cstddef>
 #include <cstdint>
 
 // One image (a texture's mip level) of a batch, as the persistent kernels see it.
@@ -46,12 +47,17 @@ struct KernelTable {
 	const void *set_scan_cap;     // k_set_scan_cap (one workgroup of 64 threads, one uint32_t argument)
 };
 
+// Row blocks (16 rows each) of a block's distance matrix that the
+// random-candidate kernels keep in registers instead of LDS.
+#define S2TC_REG_RB 2
+
 // Group memory of a four-block workgroup of the random-candidate kernels
-// with pbw row blocks: 16 pbw candidate slots, the rows from 16 on (row
-// block 0 stays in registers) and the texel features.
+// with pbw row blocks: 16 pbw candidate slots, the rows from 16 S2TC_REG_RB
+// on and the texel features.
 inline size_t normal_group_smem(const KernelTable &kt, int pbw)
 {
-	return (size_t)kt.normal_smem_cand * 16 * (size_t)pbw + (size_t)kt.normal_smem_row * 16 * (size_t)(pbw - 1) +
+	const int stored = pbw > S2TC_REG_RB ? pbw - S2TC_REG_RB : 0;
+	return (size_t)kt.normal_smem_cand * 16 * (size_t)pbw + (size_t)kt.normal_smem_row * 16 * (size_t)stored +
 	       (size_t)kt.normal_smem_fixed;
 }
 
