@@ -41,9 +41,9 @@ extern "C" {
 
 /* Random candidates per block.  Up to S2TC_LDS_NRANDOM the candidate list
  * (16 + nrandom colours, rounded up to 16, 4 bytes each), its distance rows
- * from the 17th on (64 bytes each; the first 16 stay in registers) and the
+ * from the 33rd on (64 bytes each; the first 32 stay in registers) and the
  * features of the first 16 candidates of four blocks fit the 64 KiB of LDS
- * a workgroup can allocate (224 candidates take 57,600 bytes).  Above it
+ * a workgroup can allocate (224 candidates take 53,504 bytes).  Above it
  * the same encoder keeps them in device scratch memory (under m * 68 bytes
  * per block in flight, allocated stream-ordered per call, or with a batch).  S2TC_MAX_NRANDOM
  * bounds the candidate indices to 15 bits (16 + nrandom rounded up to 16

@@ -4,7 +4,7 @@
 //
 // The only kernels of the encode that need memory beyond the source and the
 // blocks are the random-candidate kernels above S2TC_LDS_NRANDOM (candidate
-// lists, distance rows from the 17th on and column features of four blocks
+// lists, distance rows from the 33rd on and column features of four blocks
 // per workgroup, under 4 * m * 68 bytes).  Here that group memory comes from
 // `scratch`, a static device array of S2TC_NOALLOC_SCRATCH bytes that every
 // GPU holds from module load to process end; the persistent grid shrinks to
