@@ -210,14 +210,17 @@ static int resident_per_cu(const void *kernel, int block, size_t smem)
 	return n;
 }
 
-// Grid of a persistent launch: `want` workgroups, at most as many as the
-// device holds at once.  The kernels stride over their tiles by the grid
-// size, so a workgroup that has to wait for a free slot would run its whole
-// share after the others, stretching the launch.
+// Grid of a persistent launch: `want` workgroups rounded down to whole rounds
+// of what the device holds at once.  The kernels stride over their tiles by
+// the grid size, so each workgroup's share is fixed: a partial last round
+// (8 wanted where 7 fit a CU) leaves its workgroups to run whole shares after
+// the others, stretching the launch, while whole rounds (8 wanted where 2
+// fit) let the dispatcher rebalance the CUs between rounds.
 static int persistent_grid(const void *kernel, int block, size_t smem, long long want, int ncu)
 {
 	const int r = resident_per_cu(kernel, block, smem);
-	if (r > 0) want = std::min(want, (long long)r * ncu);
+	const long long cap = (long long)r * ncu;
+	if (r > 0 && want > cap) want = (want / cap) * cap;
 	return (int)std::max(1LL, want);
 }
 
