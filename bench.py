@@ -745,7 +745,7 @@ def main():
 
     serve = None
     if not a.no_serve and a.workload == "batch":
-        serve = serve_leg(s, (2000, 10000, 50000), a.serve_duration)
+        serve = serve_leg(s, (2000, 10000, 50000, 60000), a.serve_duration)
 
     configs = None
     if not a.no_configs and a.workload == "batch":
