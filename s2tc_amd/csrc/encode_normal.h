@@ -11,7 +11,8 @@
 //  * random candidates: lane t draws candidates t, t+16, ... with the
 //    counter-based generator (s2tc_prng.h) in the reference's draw order;
 //  * distance matrix: lane t computes rows i = 16p + t of dists[i][k]
-//    (:373-392) into LDS;
+//    (:373-392): the first two row blocks into its registers, the rest
+//    into LDS;
 //  * pair scan (reduce_colors_inplace, :393-410): for every row-block pair
 //    (p, q) lane t keeps its row of block p and its row of block q in
 //    registers and meets the 16 rows of block q by rotating them through the
@@ -30,7 +31,7 @@
 
 namespace s2tc {
 
-#define S2TC_RS 16  // LDS row stride in ints: no padding, so that 7 four-block workgroups of m = 80 fit a CU
+#define S2TC_RS 16  // LDS row stride in ints: no padding, so that 11 four-block workgroups of m = 80 fit a CU
 
 // Every control used here (quad_perm, row mirrors, row_ror) reads a lane of
 // the same row, and rows are all-active or all-inactive, so bound_ctrl never
