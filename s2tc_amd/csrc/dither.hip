@@ -147,8 +147,11 @@ __global__ __launch_bounds__(128) void k_dither_super_maps(DChans cs, uint8_t *_
 }
 
 // 3. start state of every super chunk: one thread per channel walks the
-// super maps (staged in LDS when they fit)
-__global__ __launch_bounds__(256) void k_dither_super_starts(DChans cs, uint8_t *__restrict__ scr, int staged)
+// super maps (staged in LDS when they fit).  With a carry (a band of a larger
+// image), the walk starts from carry[channel] when carry_in is set and leaves
+// the end state there for the next band.
+__global__ __launch_bounds__(256) void k_dither_super_starts(DChans cs, uint8_t *__restrict__ scr, int staged,
+                                                             uint8_t *__restrict__ carry, int carry_in)
 {
 	extern __shared__ uint8_t lds[];
 	const DChan ch = cs.ch[blockIdx.y];
@@ -164,10 +167,12 @@ __global__ __launch_bounds__(256) void k_dither_super_starts(DChans cs, uint8_t 
 	if (threadIdx.x != 0) return;
 	const int S = ch.nstates;
 	int st = ch.kind == 1 ? 127 : (S - 1) / 2;  // d = 0 (e = 127 for the mod-255 channel)
+	if (carry && carry_in) st = carry[blockIdx.y];
 	for (long long sc = 0; sc < cs.nsuper; ++sc) {
 		sstart[sc] = (uint8_t)st;
 		st = ch.kind == 1 ? (st + m[sc * DITHER_S]) % 255 : m[sc * DITHER_S + st];
 	}
+	if (carry) carry[blockIdx.y] = (uint8_t)st;
 }
 
 // 4. one workgroup (K threads) per super chunk: thread k finds the start
@@ -437,7 +442,7 @@ size_t rgb565_scratch_bytes(int w, int h)
 }
 
 static int run_simple(const uint8_t *src, int srccomps, long long n, DChans cs, uint8_t *out, uint8_t *scr,
-                      hipStream_t st)
+                      uint8_t *carry, int carry_in, hipStream_t st)
 {
 	simple_chunking(n, cs);
 	const unsigned nch = (unsigned)cs.nch;
@@ -446,7 +451,7 @@ static int run_simple(const uint8_t *src, int srccomps, long long n, DChans cs, 
 	k_dither_super_maps<<<dim3((unsigned)((cs.nsuper + 3) / 4), nch), 128, (size_t)4 * cs.K * DITHER_S, st>>>(cs, scr);
 	const size_t lds = (size_t)(cs.nsuper * DITHER_S);
 	const int staged = lds <= 60000;
-	k_dither_super_starts<<<dim3(1, nch), 256, staged ? lds : 0, st>>>(cs, scr, staged);
+	k_dither_super_starts<<<dim3(1, nch), 256, staged ? lds : 0, st>>>(cs, scr, staged, carry, carry_in);
 	k_dither_emit<<<dim3((unsigned)cs.nsuper, nch), (unsigned)cs.K, (size_t)cs.K * DITHER_S, st>>>(src, srccomps, n, cs,
 	                                                                                             scr, out);
 	return hip_check(hipGetLastError(), "dither SIMPLE kernels");
@@ -457,7 +462,7 @@ static int run_simple(const uint8_t *src, int srccomps, long long n, DChans cs, 
 // (:1437-1449), dither other than NONE and FLOYDSTEINBERG means SIMPLE
 // (:1419-1431).
 int launch_rgb565(const void *d_src, int srccomps, int w, int h, int alphabits, int dither, void *d_out,
-                  void *d_scratch, hipStream_t st)
+                  void *d_scratch, hipStream_t st, void *d_carry, int carry_in)
 {
 	if (srccomps != 3) srccomps = 4;
 	if (alphabits != 1 && alphabits != 4) alphabits = 8;
@@ -471,8 +476,10 @@ int launch_rgb565(const void *d_src, int srccomps, int w, int h, int alphabits, 
 		k_quantize_none<<<grid, 256, 0, st>>>(src, srccomps, n, alphabits, 1, 1, out);
 		return hip_check(hipGetLastError(), "quantize kernel");
 	}
-	if (dither == DITHER_FLOYDSTEINBERG)
+	if (dither == DITHER_FLOYDSTEINBERG) {
+		if (d_carry) return set_error(S2TC_E_ARG, "the Floyd-Steinberg dither does not run in bands");
 		return launch_rgb565_fs(src, srccomps, w, h, alphabits, out, d_scratch, st);
+	}
 	uint8_t *scr = (uint8_t *)d_scratch;
 	DChans cs;
 	cs.ch[0] = DChan{0, 0, 3, 15};
@@ -481,7 +488,7 @@ int launch_rgb565(const void *d_src, int srccomps, int w, int h, int alphabits, 
 	cs.nch = 3;
 	if (srccomps == 4 && alphabits == 1) cs.ch[cs.nch++] = DChan{3, 1, 0, 1};
 	else if (srccomps == 4 && alphabits == 4) cs.ch[cs.nch++] = DChan{3, 0, 4, 31};
-	if (int r = run_simple(src, srccomps, n, cs, out, scr, st)) return r;
+	if (int r = run_simple(src, srccomps, n, cs, out, scr, (uint8_t *)d_carry, carry_in, st)) return r;
 	if (cs.nch == 4) return 0;
 	k_quantize_none<<<grid, 256, 0, st>>>(src, srccomps, n, alphabits, 0, 1, out);
 	return hip_check(hipGetLastError(), "quantize alpha kernel");

@@ -43,10 +43,14 @@ int normal_pbw(int nrandom);
 // around the launch.
 int launch_encode(const EncodeArgs &a, hipStream_t stream, void *ext_scratch = nullptr, size_t ext_bytes = 0);
 
-// rgb565_image on the device (dither.hip); d_scratch holds rgb565_scratch_bytes(w, h)
+// rgb565_image on the device (dither.hip); d_scratch holds rgb565_scratch_bytes(w, h).
+// d_carry (4 bytes, NONE and SIMPLE only): the image is a band of rows of a
+// larger one; SIMPLE starts from the error state in d_carry when carry_in is
+// set (else from the reference's initial state) and leaves its end state
+// there for the next band.
 size_t rgb565_scratch_bytes(int w, int h);
 int launch_rgb565(const void *d_src, int srccomps, int w, int h, int alphabits, int dither, void *d_out,
-                  void *d_scratch, hipStream_t st);
+                  void *d_scratch, hipStream_t st, void *d_carry = nullptr, int carry_in = 0);
 
 inline int block_bytes(int dxt) { return dxt == DXT1 ? 8 : 16; }
 

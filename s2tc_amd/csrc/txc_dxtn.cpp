@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <mutex>
 #include <strings.h>
@@ -69,17 +70,23 @@ static const unsigned kUploadThreads = 4;
 
 struct Ctx {
 	hipStream_t stream = nullptr;
+	hipStream_t copy_stream = nullptr;    // uploads of a call encoded in bands (created on first use)
+	hipStream_t dither_stream = nullptr;  // its per-band quantization
 	DevBuf src, q, out, scratch;
 	HostBuf in_stage, in_stage2, host_out;
 	hipEvent_t stage_ev[2] = {nullptr, nullptr};  // last DMA out of in_stage / in_stage2
+	hipEvent_t band_ev = nullptr;                 // uploaded rows handed to the dither / compute stream
+	hipEvent_t dither_ev = nullptr;               // a quantized band handed to the compute stream
 };
 
 // A large upload in chunks of kStageMax bytes through the two pinned
 // buffers: while the DMA of chunk i runs, the host threads copy chunk i + 1
 // into the other buffer (one host thread alone copies pageable memory well
 // below the link rate).  A buffer is refilled only after the event of the
-// DMA that last read it.
-static int upload_large(Ctx *c, void *dst, const void *src, size_t n)
+// DMA that last read it.  The DMAs go to `st`; after each one is queued,
+// on_chunk (if given) learns how many leading bytes are then on their way.
+static int upload_large(Ctx *c, void *dst, const void *src, size_t n, hipStream_t st,
+                        const std::function<int(size_t)> &on_chunk = nullptr)
 {
 	int rc = c->in_stage.ensure(kStageMax);
 	if (!rc) rc = c->in_stage2.ensure(kStageMax);
@@ -121,8 +128,9 @@ static int upload_large(Ctx *c, void *dst, const void *src, size_t n)
 		copy_slice(i, 0);
 		while (done[i].load(std::memory_order_acquire) < nt) std::this_thread::yield();
 		const size_t off = (size_t)i * kStageMax, len = std::min(kStageMax, n - off);
-		rc = hip_check(hipMemcpyAsync((uint8_t *)dst + off, bufs[i & 1], len, hipMemcpyHostToDevice, c->stream), "H2D");
-		if (!rc) rc = hip_check(hipEventRecord(c->stage_ev[i & 1], c->stream), "hipEventRecord");
+		rc = hip_check(hipMemcpyAsync((uint8_t *)dst + off, bufs[i & 1], len, hipMemcpyHostToDevice, st), "H2D");
+		if (!rc) rc = hip_check(hipEventRecord(c->stage_ev[i & 1], st), "hipEventRecord");
+		if (!rc && on_chunk) rc = on_chunk(off + len);
 	}
 	if (rc) abort.store(true, std::memory_order_release);
 	for (auto &w : workers) w.join();
@@ -132,7 +140,7 @@ static int upload_large(Ctx *c, void *dst, const void *src, size_t n)
 // host -> device upload of a caller's buffer on the context's stream
 static int upload(Ctx *c, void *dst, const void *src, size_t n)
 {
-	if (n > kStageMax) return upload_large(c, dst, src, n);
+	if (n > kStageMax) return upload_large(c, dst, src, n, c->stream);
 	if (int r = c->in_stage.ensure(n)) return r;
 	memcpy(c->in_stage.p, src, n);
 	return hip_check(hipMemcpyAsync(dst, c->in_stage.p, n, hipMemcpyHostToDevice, c->stream), "H2D");
@@ -208,19 +216,95 @@ static EnvConfig read_env()
 	return e;
 }
 
+// Rows per band of a large call: the upload of later rows overlaps the
+// dither and encode of earlier ones.  A multiple of 4 (whole block rows).
+static const int kBandRows = 512;
+
+// Large sources (more than one staging chunk, more than one band) with the
+// NONE or SIMPLE dither: the uploads run on the copy stream, and each band of
+// kBandRows rows is quantized on the dither stream and encoded on the compute
+// stream as soon as its rows are uploaded (the SIMPLE scan of a band uses few
+// CUs for a serial walk, so it runs beside the previous band's encode).
+// SIMPLE carries its error state from band to band on the device; the encoder
+// numbers the blocks of a band from the band's first block, so the random
+// candidates are those of the whole image.
+static int encode_host_bands(Ctx *c, int srccomps, int width, int height, const uint8_t *src, int dxt,
+                             const EncodeArgs &base, int dither, int alphabits)
+{
+	const size_t row_bytes = (size_t)width * srccomps;
+	const size_t n = (size_t)width * height;
+	const int bw = (width + 3) / 4, bb = block_bytes(dxt);
+	const int nbands = (height + kBandRows - 1) / kBandRows;
+	const bool fused = dither == DITHER_NONE && srccomps == 4;
+	if (!fused) {
+		if (int r = c->q.ensure(n * 4)) return r;
+		if (int r = c->scratch.ensure(rgb565_scratch_bytes(width, kBandRows) + 256)) return r;
+	}
+	if (!c->copy_stream)
+		if (int r = hip_check(hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking), "hipStreamCreate"))
+			return r;
+	if (!c->dither_stream)
+		if (int r = hip_check(hipStreamCreateWithFlags(&c->dither_stream, hipStreamNonBlocking), "hipStreamCreate"))
+			return r;
+	if (!c->band_ev)
+		if (int r = hip_check(hipEventCreateWithFlags(&c->band_ev, hipEventDisableTiming), "hipEventCreate"))
+			return r;
+	if (!c->dither_ev)
+		if (int r = hip_check(hipEventCreateWithFlags(&c->dither_ev, hipEventDisableTiming), "hipEventCreate"))
+			return r;
+	uint8_t *carry = (uint8_t *)c->scratch.p + rgb565_scratch_bytes(width, kBandRows);
+	int next = 0;
+	auto on_chunk = [&](size_t uploaded) -> int {
+		bool waited = false;
+		while (next < nbands && (size_t)std::min(height, (next + 1) * kBandRows) * row_bytes <= uploaded) {
+			hipStream_t first = fused ? c->stream : c->dither_stream;  // the band's first user
+			if (!waited) {
+				if (int r = hip_check(hipEventRecord(c->band_ev, c->copy_stream), "hipEventRecord")) return r;
+				if (int r = hip_check(hipStreamWaitEvent(first, c->band_ev, 0), "hipStreamWaitEvent")) return r;
+				waited = true;
+			}
+			const int r0 = next * kBandRows, rows = std::min(kBandRows, height - r0);
+			const uint8_t *bsrc = (const uint8_t *)c->src.p + (size_t)r0 * row_bytes;
+			EncodeArgs a = base;
+			a.height = rows;
+			a.block_index_base = (uint32_t)((size_t)(r0 / 4) * bw);
+			a.dst = (uint8_t *)c->out.p + (size_t)(r0 / 4) * bw * bb;
+			if (fused) {
+				a.src = bsrc;
+				a.src_kind = S2TC_SRC_RGBA8;
+			} else {
+				uint8_t *bq = (uint8_t *)c->q.p + (size_t)r0 * width * 4;
+				if (int r = launch_rgb565(bsrc, srccomps, width, rows, alphabits, dither, bq, c->scratch.p,
+				                          c->dither_stream, carry, next > 0))
+					return r;
+				if (int r = hip_check(hipEventRecord(c->dither_ev, c->dither_stream), "hipEventRecord")) return r;
+				if (int r = hip_check(hipStreamWaitEvent(c->stream, c->dither_ev, 0), "hipStreamWaitEvent")) return r;
+				a.src = bq;
+				a.src_kind = S2TC_SRC_QUANTIZED;
+			}
+			if (int r = launch_encode(a, c->stream)) return r;
+			++next;
+		}
+		return 0;
+	};
+	if (int r = upload_large(c, c->src.p, src, n * srccomps, c->copy_stream, on_chunk)) return r;
+	if (next != nbands) return set_error(S2TC_E_ARG, "banded encode: %d of %d bands launched", next, nbands);
+	return 0;
+}
+
 // Whole-image encode of host data on the context's stream; writes the
 // blocks in raster order to c->host_out.
 static int encode_host_image(Ctx *c, int srccomps, int width, int height, const uint8_t *src, int dxt, int cd,
                              int nrandom, int refine, int dither, uint64_t seed)
 {
 	if (srccomps != 3) srccomps = 4;
+	if (dither != DITHER_NONE && dither != DITHER_FLOYDSTEINBERG) dither = DITHER_SIMPLE;
 	const size_t n = (size_t)width * height;
 	const size_t nb = (size_t)((width + 3) / 4) * ((height + 3) / 4);
 	const size_t obytes = nb * block_bytes(dxt);
 	const int alphabits = dxt == DXT1 ? 1 : dxt == DXT3 ? 4 : 8;
 	if (int r = c->src.ensure(n * srccomps)) return r;
 	if (int r = c->out.ensure(obytes)) return r;
-	if (int r = upload(c, c->src.p, src, n * srccomps)) return r;
 	EncodeArgs a{};
 	a.dst = c->out.p;
 	a.width = width;
@@ -230,19 +314,24 @@ static int encode_host_image(Ctx *c, int srccomps, int width, int height, const 
 	a.nrandom = nrandom;
 	a.refine = refine;
 	a.seed = seed;
-	if (dither == DITHER_NONE && srccomps == 4) {
-		a.src = c->src.p;  // quantization fused into the encoder's loads
-		a.src_kind = S2TC_SRC_RGBA8;
+	if (dither != DITHER_FLOYDSTEINBERG && height > kBandRows && n * srccomps > kStageMax) {
+		if (int r = encode_host_bands(c, srccomps, width, height, src, dxt, a, dither, alphabits)) return r;
 	} else {
-		if (int r = c->q.ensure(n * 4)) return r;
-		if (int r = c->scratch.ensure(rgb565_scratch_bytes(width, height))) return r;
-		if (int r = launch_rgb565(c->src.p, srccomps, width, height, alphabits, dither, c->q.p, c->scratch.p,
-		                          c->stream))
-			return r;
-		a.src = c->q.p;
-		a.src_kind = S2TC_SRC_QUANTIZED;
+		if (int r = upload(c, c->src.p, src, n * srccomps)) return r;
+		if (dither == DITHER_NONE && srccomps == 4) {
+			a.src = c->src.p;  // quantization fused into the encoder's loads
+			a.src_kind = S2TC_SRC_RGBA8;
+		} else {
+			if (int r = c->q.ensure(n * 4)) return r;
+			if (int r = c->scratch.ensure(rgb565_scratch_bytes(width, height))) return r;
+			if (int r = launch_rgb565(c->src.p, srccomps, width, height, alphabits, dither, c->q.p, c->scratch.p,
+			                          c->stream))
+				return r;
+			a.src = c->q.p;
+			a.src_kind = S2TC_SRC_QUANTIZED;
+		}
+		if (int r = launch_encode(a, c->stream)) return r;
 	}
-	if (int r = launch_encode(a, c->stream)) return r;
 	if (int r = c->host_out.ensure(obytes)) return r;
 	if (int r = hip_check(hipMemcpyAsync(c->host_out.data(), c->out.p, obytes, hipMemcpyDeviceToHost, c->stream),
 	                      "D2H"))

@@ -232,6 +232,39 @@ def test_large_call_ragged_chunks_against_oracle(gpu, oracle, clean_env, w, h, s
         assert np.array_equal(dst, exp[:dst.size]), (w, h, sc, fmt)
 
 
+@pytest.mark.parametrize("w,h,sc,dither,cd,nr,refine", [
+    (1536, 1400, 4, 0, 5, -1, 1),    # NONE, RGBA: quantized in the encoder's loads
+    (2048, 1500, 3, 0, 4, -1, 2),    # NONE, RGB: quantized per band
+    (1536, 1401, 4, 1, 1, 8, 2),     # SIMPLE carried over bands, random candidates across bands
+    (2048, 1403, 3, 1, 3, 0, 1),     # SIMPLE, RGB, all 16 candidates
+])
+def test_large_call_bands_against_oracle(gpu, oracle, clean_env, w, h, sc, dither, cd, nr, refine):
+    """Calls above one staging chunk and one band of rows are encoded band by
+    band while later rows upload: the SIMPLE dither's error state crosses the
+    band boundaries on the device, each band's blocks keep their whole-image
+    indices for the random candidates, and the last band is ragged (not a
+    multiple of 4 rows); the output equals the oracle's tx_compress_dxtn."""
+    L = gpu.lib()
+    L.tx_compress_dxtn.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint,
+                                   ctypes.c_void_p, ctypes.c_int]
+    os.environ.update(S2TC_DITHER_MODE=DITHERS[dither], S2TC_COLORDIST_MODE=NAMES[cd], S2TC_RANDOM_COLORS=str(nr),
+                      S2TC_REFINE_COLORS=REFINES[refine], S2TC_RANDOM_SEED="7")
+    rng = np.random.default_rng(w + 3 * h + sc)
+    yy, xx = np.mgrid[0:h, 0:w]
+    base = np.stack([(xx * 255) // w, (yy * 255) // h, ((xx + yy) * 255) // (w + h), (xx * yy) % 256], -1)
+    src = np.clip(base[..., :sc] + rng.integers(-12, 13, (h, w, sc)), 0, 255).astype(np.uint8).reshape(-1)
+    assert src.size > 8 << 20
+    bw, bh = (w + 3) // 4, (h + 3) // 4
+    for fmt, bs in ((0x83F0, 8), (0x83F3, 16)):
+        _say(f"banded tx_compress_dxtn {w}x{h} srccomps={sc} dither={DITHERS[dither]} cd={NAMES[cd]} "
+             f"nrandom={nr} refine={REFINES[refine]} format={fmt:#x}")
+        dst = np.zeros(bw * bh * bs, np.uint8)
+        L.tx_compress_dxtn(sc, w, h, src.ctypes.data, fmt, dst.ctypes.data, bw * bs)
+        r, exp = oracle.compress(sc, w, h, src, fmt, bw * bs, cd, nr, refine, dither, 7)
+        assert r == 0
+        assert np.array_equal(dst, exp[:dst.size]), (w, h, sc, dither, cd, nr, fmt)
+
+
 def test_large_call_staging_and_row_gaps(gpu, clean_env):
     """Sources above the single staging buffer stream through chunked pinned
     buffers, and large outputs are laid out by several host threads: a
