@@ -292,10 +292,26 @@ static int encode_host_bands(Ctx *c, int srccomps, int width, int height, const 
 	return 0;
 }
 
+static int encode_host_image_on(Ctx *c, int srccomps, int width, int height, const uint8_t *src, int dxt, int cd,
+                                int nrandom, int refine, int dither, uint64_t seed);
+
 // Whole-image encode of host data on the context's stream; writes the
-// blocks in raster order to c->host_out.
+// blocks in raster order to c->host_out.  On an error every stream of the
+// context is drained before the context goes back to the pool, so that no
+// launch of this call still uses its buffers when the next call resizes them.
 static int encode_host_image(Ctx *c, int srccomps, int width, int height, const uint8_t *src, int dxt, int cd,
                              int nrandom, int refine, int dither, uint64_t seed)
+{
+	const int rc = encode_host_image_on(c, srccomps, width, height, src, dxt, cd, nrandom, refine, dither, seed);
+	if (rc) {
+		for (hipStream_t st : {c->stream, c->copy_stream, c->dither_stream})
+			if (st) (void)hipStreamSynchronize(st);
+	}
+	return rc;
+}
+
+static int encode_host_image_on(Ctx *c, int srccomps, int width, int height, const uint8_t *src, int dxt, int cd,
+                                int nrandom, int refine, int dither, uint64_t seed)
 {
 	if (srccomps != 3) srccomps = 4;
 	if (dither != DITHER_NONE && dither != DITHER_FLOYDSTEINBERG) dither = DITHER_SIMPLE;
