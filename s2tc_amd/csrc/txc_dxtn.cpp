@@ -72,11 +72,13 @@ struct Ctx {
 	hipStream_t stream = nullptr;
 	hipStream_t copy_stream = nullptr;    // uploads of a call encoded in bands (created on first use)
 	hipStream_t dither_stream = nullptr;  // its per-band quantization
+	hipStream_t d2h_stream = nullptr;     // and the download of each encoded band
 	DevBuf src, q, out, scratch;
 	HostBuf in_stage, in_stage2, host_out;
 	hipEvent_t stage_ev[2] = {nullptr, nullptr};  // last DMA out of in_stage / in_stage2
 	hipEvent_t band_ev = nullptr;                 // uploaded rows handed to the dither / compute stream
 	hipEvent_t dither_ev = nullptr;               // a quantized band handed to the compute stream
+	hipEvent_t enc_ev = nullptr;                  // an encoded band handed to the download stream
 };
 
 // A large upload in chunks of kStageMax bytes through the two pinned
@@ -224,7 +226,8 @@ static const int kBandRows = 512;
 // NONE or SIMPLE dither: the uploads run on the copy stream, and each band of
 // kBandRows rows is quantized on the dither stream and encoded on the compute
 // stream as soon as its rows are uploaded (the SIMPLE scan of a band uses few
-// CUs for a serial walk, so it runs beside the previous band's encode).
+// CUs for a serial walk, so it runs beside the previous band's encode); each
+// band's blocks download on a fourth stream, beside the later uploads.
 // SIMPLE carries its error state from band to band on the device; the encoder
 // numbers the blocks of a band from the band's first block, so the random
 // candidates are those of the whole image.
@@ -246,6 +249,14 @@ static int encode_host_bands(Ctx *c, int srccomps, int width, int height, const 
 	if (!c->dither_stream)
 		if (int r = hip_check(hipStreamCreateWithFlags(&c->dither_stream, hipStreamNonBlocking), "hipStreamCreate"))
 			return r;
+	if (!c->d2h_stream)
+		if (int r = hip_check(hipStreamCreateWithFlags(&c->d2h_stream, hipStreamNonBlocking), "hipStreamCreate"))
+			return r;
+	if (!c->enc_ev)
+		if (int r = hip_check(hipEventCreateWithFlags(&c->enc_ev, hipEventDisableTiming), "hipEventCreate"))
+			return r;
+	const size_t obytes = (size_t)bw * ((height + 3) / 4) * bb;
+	if (int r = c->host_out.ensure(obytes)) return r;
 	if (!c->band_ev)
 		if (int r = hip_check(hipEventCreateWithFlags(&c->band_ev, hipEventDisableTiming), "hipEventCreate"))
 			return r;
@@ -283,13 +294,20 @@ static int encode_host_bands(Ctx *c, int srccomps, int width, int height, const 
 				a.src_kind = S2TC_SRC_QUANTIZED;
 			}
 			if (int r = launch_encode(a, c->stream)) return r;
+			const size_t o0 = (size_t)(r0 / 4) * bw * bb, o1 = std::min(obytes, (size_t)((r0 + rows + 3) / 4) * bw * bb);
+			if (int r = hip_check(hipEventRecord(c->enc_ev, c->stream), "hipEventRecord")) return r;
+			if (int r = hip_check(hipStreamWaitEvent(c->d2h_stream, c->enc_ev, 0), "hipStreamWaitEvent")) return r;
+			if (int r = hip_check(hipMemcpyAsync(c->host_out.data() + o0, (uint8_t *)c->out.p + o0, o1 - o0,
+			                                     hipMemcpyDeviceToHost, c->d2h_stream),
+			                      "D2H"))
+				return r;
 			++next;
 		}
 		return 0;
 	};
 	if (int r = upload_large(c, c->src.p, src, n * srccomps, c->copy_stream, on_chunk)) return r;
 	if (next != nbands) return set_error(S2TC_E_ARG, "banded encode: %d of %d bands launched", next, nbands);
-	return 0;
+	return hip_check(hipStreamSynchronize(c->d2h_stream), "hipStreamSynchronize");
 }
 
 static int encode_host_image_on(Ctx *c, int srccomps, int width, int height, const uint8_t *src, int dxt, int cd,
@@ -304,7 +322,7 @@ static int encode_host_image(Ctx *c, int srccomps, int width, int height, const 
 {
 	const int rc = encode_host_image_on(c, srccomps, width, height, src, dxt, cd, nrandom, refine, dither, seed);
 	if (rc) {
-		for (hipStream_t st : {c->stream, c->copy_stream, c->dither_stream})
+		for (hipStream_t st : {c->stream, c->copy_stream, c->dither_stream, c->d2h_stream})
 			if (st) (void)hipStreamSynchronize(st);
 	}
 	return rc;
@@ -331,7 +349,8 @@ static int encode_host_image_on(Ctx *c, int srccomps, int width, int height, con
 	a.refine = refine;
 	a.seed = seed;
 	if (dither != DITHER_FLOYDSTEINBERG && height > kBandRows && n * srccomps > kStageMax) {
-		if (int r = encode_host_bands(c, srccomps, width, height, src, dxt, a, dither, alphabits)) return r;
+		// the blocks are in c->host_out once the last band's download is done
+		return encode_host_bands(c, srccomps, width, height, src, dxt, a, dither, alphabits);
 	} else {
 		if (int r = upload(c, c->src.p, src, n * srccomps)) return r;
 		if (dither == DITHER_NONE && srccomps == 4) {
