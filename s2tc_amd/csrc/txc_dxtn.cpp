@@ -218,14 +218,21 @@ static EnvConfig read_env()
 	return e;
 }
 
-// Rows per band of a large call: the upload of later rows overlaps the
-// dither and encode of earlier ones.  A multiple of 4 (whole block rows).
-static const int kBandRows = 512;
+// Rows per band of a large call (a multiple of 4: whole block rows): the
+// upload of later rows overlaps the dither and encode of earlier ones.  At
+// least 512 rows, and at least 2^18 blocks per band, which the one-lane-per-
+// block encoders need to fill the GPU (measured: 8192-wide normal map best at
+// 512 rows, 4096-wide tiles at 1024).
+static int band_rows(int width)
+{
+	const long long r = ((1LL << 22) + width - 1) / width;
+	return (int)std::max(512LL, (r + 3) & ~3LL);
+}
 
 // Large sources (more than one staging chunk, more than one band) with the
 // NONE or SIMPLE dither: the uploads run on the copy stream, and each band of
-// kBandRows rows is quantized on the dither stream and encoded on the compute
-// stream as soon as its rows are uploaded (the SIMPLE scan of a band uses few
+// band_rows(width) rows is quantized on the dither stream and encoded on the
+// compute stream as soon as its rows are uploaded (the SIMPLE scan of a band uses few
 // CUs for a serial walk, so it runs beside the previous band's encode); each
 // band's blocks download on a fourth stream, beside the later uploads.
 // SIMPLE carries its error state from band to band on the device; the encoder
@@ -234,14 +241,15 @@ static const int kBandRows = 512;
 static int encode_host_bands(Ctx *c, int srccomps, int width, int height, const uint8_t *src, int dxt,
                              const EncodeArgs &base, int dither, int alphabits)
 {
+	const int brows = band_rows(width);
 	const size_t row_bytes = (size_t)width * srccomps;
 	const size_t n = (size_t)width * height;
 	const int bw = (width + 3) / 4, bb = block_bytes(dxt);
-	const int nbands = (height + kBandRows - 1) / kBandRows;
+	const int nbands = (height + brows - 1) / brows;
 	const bool fused = dither == DITHER_NONE && srccomps == 4;
 	if (!fused) {
 		if (int r = c->q.ensure(n * 4)) return r;
-		if (int r = c->scratch.ensure(rgb565_scratch_bytes(width, kBandRows) + 256)) return r;
+		if (int r = c->scratch.ensure(rgb565_scratch_bytes(width, brows) + 256)) return r;
 	}
 	if (!c->copy_stream)
 		if (int r = hip_check(hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking), "hipStreamCreate"))
@@ -263,18 +271,18 @@ static int encode_host_bands(Ctx *c, int srccomps, int width, int height, const 
 	if (!c->dither_ev)
 		if (int r = hip_check(hipEventCreateWithFlags(&c->dither_ev, hipEventDisableTiming), "hipEventCreate"))
 			return r;
-	uint8_t *carry = (uint8_t *)c->scratch.p + rgb565_scratch_bytes(width, kBandRows);
+	uint8_t *carry = (uint8_t *)c->scratch.p + rgb565_scratch_bytes(width, brows);
 	int next = 0;
 	auto on_chunk = [&](size_t uploaded) -> int {
 		bool waited = false;
-		while (next < nbands && (size_t)std::min(height, (next + 1) * kBandRows) * row_bytes <= uploaded) {
+		while (next < nbands && (size_t)std::min(height, (next + 1) * brows) * row_bytes <= uploaded) {
 			hipStream_t first = fused ? c->stream : c->dither_stream;  // the band's first user
 			if (!waited) {
 				if (int r = hip_check(hipEventRecord(c->band_ev, c->copy_stream), "hipEventRecord")) return r;
 				if (int r = hip_check(hipStreamWaitEvent(first, c->band_ev, 0), "hipStreamWaitEvent")) return r;
 				waited = true;
 			}
-			const int r0 = next * kBandRows, rows = std::min(kBandRows, height - r0);
+			const int r0 = next * brows, rows = std::min(brows, height - r0);
 			const uint8_t *bsrc = (const uint8_t *)c->src.p + (size_t)r0 * row_bytes;
 			EncodeArgs a = base;
 			a.height = rows;
@@ -348,7 +356,7 @@ static int encode_host_image_on(Ctx *c, int srccomps, int width, int height, con
 	a.nrandom = nrandom;
 	a.refine = refine;
 	a.seed = seed;
-	if (dither != DITHER_FLOYDSTEINBERG && height > kBandRows && n * srccomps > kStageMax) {
+	if (dither != DITHER_FLOYDSTEINBERG && height > band_rows(width) && n * srccomps > kStageMax) {
 		// the blocks are in c->host_out once the last band's download is done
 		return encode_host_bands(c, srccomps, width, height, src, dxt, a, dither, alphabits);
 	} else {

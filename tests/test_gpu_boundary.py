@@ -233,17 +233,18 @@ def test_large_call_ragged_chunks_against_oracle(gpu, oracle, clean_env, w, h, s
 
 
 @pytest.mark.parametrize("w,h,sc,dither,cd,nr,refine", [
-    (1536, 1400, 4, 0, 5, -1, 1),    # NONE, RGBA: quantized in the encoder's loads
-    (2048, 1500, 3, 0, 4, -1, 2),    # NONE, RGB: quantized per band
-    (1536, 1401, 4, 1, 1, 8, 2),     # SIMPLE carried over bands, random candidates across bands
-    (2048, 1403, 3, 1, 3, 0, 1),     # SIMPLE, RGB, all 16 candidates
+    (4096, 1100, 4, 0, 5, -1, 1),    # NONE, RGBA: quantized in the encoder's loads; bands of 1024 rows
+    (8192, 1300, 3, 0, 4, -1, 2),    # NONE, RGB: quantized per band; three bands of 512 rows
+    (4096, 1401, 4, 1, 1, 8, 2),     # SIMPLE carried over bands, random candidates across bands
+    (4096, 1403, 3, 1, 3, 0, 1),     # SIMPLE, RGB, all 16 candidates
 ])
 def test_large_call_bands_against_oracle(gpu, oracle, clean_env, w, h, sc, dither, cd, nr, refine):
-    """Calls above one staging chunk and one band of rows are encoded band by
-    band while later rows upload: the SIMPLE dither's error state crosses the
-    band boundaries on the device, each band's blocks keep their whole-image
-    indices for the random candidates, and the last band is ragged (not a
-    multiple of 4 rows); the output equals the oracle's tx_compress_dxtn."""
+    """Calls above one staging chunk and one band of rows (max(512, 2^22 /
+    width) rows) are encoded band by band while later rows upload: the SIMPLE
+    dither's error state crosses the band boundaries on the device, each
+    band's blocks keep their whole-image indices for the random candidates,
+    and the last band is ragged (not a multiple of 4 rows); the output equals
+    the oracle's tx_compress_dxtn."""
     L = gpu.lib()
     L.tx_compress_dxtn.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint,
                                    ctypes.c_void_p, ctypes.c_int]
