@@ -225,7 +225,9 @@ typedef struct {
 	int streams;             /* batches in flight at once (1..16), each on its own stream */
 	int max_batch;           /* most requests in one dispatched batch */
 	int serial_plain;        /* 1: one request at a time with one plain launch per image */
-	int wg_per_cu;           /* persistent grid of the class kernels (0: one workgroup per tile) */
+	int wg_per_cu;           /* persistent grid of the class kernels, rounded down to whole rounds of the
+	                            workgroups a CU holds (0: one workgroup per tile); 16 serves the mixed
+	                            pool best */
 	void **keep_outputs;     /* NULL, or one host buffer per job receiving its blocks */
 } s2tc_amd_serve_opts;
 

@@ -474,7 +474,7 @@ class ServeStats(ctypes.Structure):
                 ("p99_ms", ctypes.c_double), ("mean_ms", ctypes.c_double), ("max_ms", ctypes.c_double)]
 
 
-def serve(images, requests, hbm_budget=64 << 30, streams=4, max_batch=256, serial_plain=False, wg_per_cu=8,
+def serve(images, requests, hbm_budget=64 << 30, streams=4, max_batch=256, serial_plain=False, wg_per_cu=16,
           keep_outputs=None):
     """Run requests [(arrival_s, first_job, njobs)] over the images of a
     DeviceImages in real time.  Returns (stats dict, latencies_ms array).
