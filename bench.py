@@ -324,6 +324,39 @@ def quick_leg(s, size=16384, reps=10, cds=(0, 1, 2, 3, 4, 5, 6)):
     return out
 
 
+def block_call_row(s, ref, reps=200):
+    """The per-block entry points of s2tc_encode_block_func (one 4x4 block of
+    quantized texels per call, synchronous: one upload, launch and download
+    each), median over `reps` calls, beside the compiled reference's own
+    per-block function on the same block."""
+    import numpy as np
+    FT = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                          ctypes.c_int)
+    rng = np.random.default_rng(3)
+    q = np.stack([rng.integers(0, 32, 16), rng.integers(0, 64, 16), rng.integers(0, 32, 16),
+                  rng.integers(0, 256, 16)], -1).astype(np.uint8).reshape(-1)
+    out = np.zeros(16, np.uint8)
+
+    def time_lib(lib):
+        lib.s2tc_encode_block_func.restype = ctypes.c_void_p
+        lib.s2tc_encode_block_func.argtypes = [ctypes.c_int] * 4
+        f = FT(lib.s2tc_encode_block_func(2, 5, -1, 1))  # DXT5, WAVG, quick selection, REFINE_ALWAYS
+        for _ in range(10):
+            f(out.ctypes.data, q.ctypes.data, 4, 4, 4, -1)
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            f(out.ctypes.data, q.ctypes.data, 4, 4, 4, -1)
+            ts.append(time.perf_counter() - t0)
+        return round(float(np.median(ts)) * 1e6, 2)
+    row = dict(workload="one 4x4 block, DXT5, WAVG, quick, REFINE_ALWAYS", entry="s2tc_encode_block_func",
+               texels=16, us_per_call=time_lib(s.lib()), calls=reps)
+    if ref is not None:
+        row["cpu_reference_us_per_call"] = time_lib(ref.lib)
+    log(f"calls {row}")
+    return row
+
+
 def calls_leg(s, with_cpu=True, which=("sinus256", "tiles4096", "normal8192")):
     """The drop-in entry point tx_compress_dxtn on the single-image workloads,
     timed end to end per call (host buffers in and out: upload, SIMPLE dither,
@@ -382,6 +415,7 @@ def calls_leg(s, with_cpu=True, which=("sinus256", "tiles4096", "normal8192")):
                 row["cpu_reference_us_per_call"] = round(float(np.median(rs)) * 1e6, 1)
             log(f"calls {row}")
             out.append(row)
+        out.append(block_call_row(s, ref))
     finally:
         for k, v in saved.items():
             if v is None:
