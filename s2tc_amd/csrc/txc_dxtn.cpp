@@ -79,6 +79,10 @@ struct Ctx {
 	hipEvent_t band_ev = nullptr;                 // uploaded rows handed to the dither / compute stream
 	hipEvent_t dither_ev = nullptr;               // a quantized band handed to the compute stream
 	hipEvent_t enc_ev = nullptr;                  // an encoded band handed to the download stream
+	// per-block calls: one mapped pinned page that the kernel reads the block's
+	// texels from (offset 0) and writes its encoding to (offset 64)
+	uint8_t *blk_host = nullptr;
+	void *blk_dev = nullptr;
 };
 
 // A large upload in chunks of kStageMax bytes through the two pinned
@@ -397,31 +401,36 @@ static void encode_one_block(unsigned char *out, const unsigned char *rgba, int 
 		return;
 	}
 	if (w < 1 || h < 1 || w > 4 || h > 4) return;
-	uint8_t tile[64];
-	for (int y = 0; y < h; ++y) memcpy(tile + y * w * 4, rgba + (size_t)y * iw * 4, (size_t)w * 4);
 	const int bs = block_bytes(dxt);
-	int r = c->src.ensure(64);
-	if (!r) r = c->out.ensure(16);
-	if (!r) r = hip_check(hipMemcpyAsync(c->src.p, tile, (size_t)w * h * 4, hipMemcpyHostToDevice, c->stream), "H2D");
+	int r = 0;
+	if (!c->blk_host) {
+		r = hip_check(hipHostMalloc((void **)&c->blk_host, 4096, hipHostMallocMapped), "hipHostMalloc");
+		if (!r) r = hip_check(hipHostGetDevicePointer(&c->blk_dev, c->blk_host, 0), "hipHostGetDevicePointer");
+		if (r && c->blk_host) {
+			(void)hipHostFree(c->blk_host);
+			c->blk_host = nullptr;
+		}
+	}
 	if (!r) {
+		// the block's rows, packed, where the kernel reads them over the bus
+		for (int y = 0; y < h; ++y) memcpy(c->blk_host + y * w * 4, rgba + (size_t)y * iw * 4, (size_t)w * 4);
 		// A one-block image whose only block has index bidx in the generator.
 		// The selection-time mode is kept: quick mode runs with nrandom < 0,
 		// MODE_NORMAL with max(nrandom, 0) extra candidates.
-		EncodeArgs a{c->src.p, c->out.p, S2TC_SRC_QUANTIZED, w, h, dxt, cd,
-		             fast ? -1 : (nrandom > 0 ? nrandom : 0), refine, 0, 0};
+		uint8_t *dev = (uint8_t *)c->blk_dev;
+		EncodeArgs a{dev, dev + 64, S2TC_SRC_QUANTIZED, w, h, dxt, cd, fast ? -1 : (nrandom > 0 ? nrandom : 0),
+		             refine, 0, 0};
 		const char *sv = getenv("S2TC_RANDOM_SEED");
 		a.seed = sv ? strtoull(sv, nullptr, 0) : 0;
 		a.block_index_base = bidx;
 		r = launch_encode(a, c->stream);
 	}
-	uint8_t blk[16];
-	if (!r) r = hip_check(hipMemcpyAsync(blk, c->out.p, bs, hipMemcpyDeviceToHost, c->stream), "D2H");
 	if (!r) r = hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
 	if (r) {
 		fprintf(stderr, "s2tc: %s\n", s2tc_amd_last_error());
 		return;
 	}
-	memcpy(out, blk, bs);
+	memcpy(out, c->blk_host + 64, bs);
 }
 
 template <int DXT, int CD, int FAST, int REFINE>
