@@ -587,13 +587,20 @@ S2TC_D void encode_block_normal(uint32_t tex, int t, int w, int h, int dxt, int 
 		lr = maxr - minr + 1, lg = maxg - ming + 1, lb = maxb - minb + 1;
 		const int la = maxa - mina + 1;
 		const int dpc = dxt == DXT5 ? 4 : 3;
+		// a channel whose range is one value takes it for every draw (draw % 1
+		// is 0): blocks of one colour (and of one alpha) skip those draws
 		for (int c = t; c < nrandom; c += 16) {
 			const uint32_t k = (uint32_t)(c * dpc);
-			const uint32_t cr = (uint32_t)(minr + (int)((uint32_t)s2tc_prng_draw(bkey, k) % (uint32_t)lr));
-			const uint32_t cg = (uint32_t)(ming + (int)((uint32_t)s2tc_prng_draw(bkey, k + 1) % (uint32_t)lg));
-			const uint32_t cb = (uint32_t)(minb + (int)((uint32_t)s2tc_prng_draw(bkey, k + 2) % (uint32_t)lb));
-			uint32_t ca = 0;
-			if (dxt == DXT5) ca = (uint32_t)(mina + (int)((uint32_t)s2tc_prng_draw(bkey, k + 3) % (uint32_t)la));
+			uint32_t cr = (uint32_t)minr, cg = (uint32_t)ming, cb = (uint32_t)minb, ca = 0;
+			if (!cuni) {
+				cr += (uint32_t)s2tc_prng_draw(bkey, k) % (uint32_t)lr;
+				cg += (uint32_t)s2tc_prng_draw(bkey, k + 1) % (uint32_t)lg;
+				cb += (uint32_t)s2tc_prng_draw(bkey, k + 2) % (uint32_t)lb;
+			}
+			if (dxt == DXT5) {
+				ca = (uint32_t)mina;
+				if (la > 1) ca += (uint32_t)s2tc_prng_draw(bkey, k + 3) % (uint32_t)la;
+			}
 			sm.cand[n + c] = (cr << 11) | (cg << 5) | cb | (ca << 16);
 		}
 		m = n + nrandom;
